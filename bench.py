@@ -1,0 +1,132 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 Fock build (J+K) of the 16-atom random cluster on one MI355X.
+
+A "step" is one full J+K build with the density, J and K resident in HBM.  The workload is the
+seeded 16-atom random cluster: heavy atoms 3s2p1d and H 2s1p, each shell a 3-primitive
+contraction.  Odd electron counts are evened by removing one H, which leaves 15 atoms here.
+
+``value`` is ERI quartets/s: unique contracted shell quartets (8-fold symmetry) computed per
+second, counted over the timed steps.  The JSON line also carries:
+
+* ``roofline``: the dominant class kernel.  Its algorithmic FLOPs per launch are divided by its
+  average launch time, taken from HIP events on the engine stream over the timed region.
+* ``cpu_baseline``: the CPU oracle's direct J/K, timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+METRIC = "Fock-build s & ERI quartets/s; SCF time-to-1e-6 Eh: mixed+learned vs FP64 GPU"
+
+
+def cpu_baseline(mol, D, budget_s=15.0, threads=None):
+    """Oracle direct J/K (OpenMP) on the first bra shell pairs of the same workload, scaled to quartets/s."""
+    import oracle
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    ob = oracle.Basis(mol["shells"])
+    # calibrate on a small prefix, then size the sample to ~budget_s
+    limit = 64
+    t0 = time.perf_counter()
+    _, _, cnt = ob.jk_direct(D, nthreads=threads, pair_limit=limit)
+    dt = time.perf_counter() - t0
+    ns = len(mol["shells"])
+    npairs = ns * (ns + 1) // 2
+    rate_pairs = max(dt, 1e-3) / limit
+    # quartets grow ~ pair index, so scale the prefix length by sqrt of the time ratio
+    limit2 = int(min(npairs, limit * max(1.0, (budget_s / max(dt, 1e-3)) ** 0.5)))
+    t0 = time.perf_counter()
+    _, _, cnt = ob.jk_direct(D, nthreads=threads, pair_limit=limit2)
+    dt = time.perf_counter() - t0
+    del rate_pairs
+    return {"value": cnt / dt, "unit": "quartets/s", "cores": threads, "kind": "port",
+            "sample": "oracle direct J+K (8-fold, FP64, OpenMP) over the first %d of %d bra shell pairs "
+                      "(%d contracted quartets, %.1f s)" % (limit2, npairs, cnt, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; the multi-GPU path is design-only (--gpus %d requested)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import qchem_amd as qa
+    from qchem_amd import systems as S
+
+    mol = S.random_cluster()
+    B = qa.Basis(mol["shells"])
+    eng = qa.JKEngine(B, device=0)
+    n = B.nbf
+    rng = np.random.default_rng(7)
+    C = rng.normal(size=(n, sum(mol["Z"]) // 2)) / np.sqrt(n)
+    Dh = 2.0 * C @ C.T
+    dev = torch.device("cuda:0")
+    D = torch.from_numpy(Dh).to(dev)
+    J = torch.empty_like(D)
+    K = torch.empty_like(D)
+    stream = torch.cuda.current_stream(dev)
+
+    flags = qa.QCJK_J | qa.QCJK_K
+    st = eng.build_device(D, J, K, flags, stream=stream, stats=True)  # counts + first build
+    for _ in range(args.warmup):
+        eng.build_device(D, J, K, flags, stream=stream)
+    torch.cuda.synchronize()
+    eng.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.build_device(D, J, K, flags, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    rep = eng.class_report()
+    eng.profile(False)
+    elapsed = t1 - t0
+    ms = 1e3 * elapsed / args.steps
+    quartets = st.quartets_computed
+    value = quartets * args.steps / elapsed
+
+    dom = max(rep, key=lambda r: r["device_ms"])
+    launches = max(dom["launches"], 1)
+    per_launch_flops = dom["flops"] / launches
+    per_launch_s = dom["device_ms"] / launches / 1e3
+    achieved = per_launch_flops / per_launch_s / 1e12
+    total_flops = sum(r["flops"] for r in rep) / max(1, args.steps)
+    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "k_jk_%s<%d,%d,%d,%d>" % ("heavy" if dom["wavefront_per_quartet"] else "light",
+                                                dom["la"], dom["lb"], dom["lc"], dom["ld"]),
+            "kernel_ms_per_launch": per_launch_s * 1e3, "kernel_flops_per_launch": per_launch_flops,
+            "build_tflops": total_flops / (ms / 1e3) / 1e12}
+
+    cpu = None if args.no_cpu else cpu_baseline(mol, Dh)
+    out = {"metric": METRIC, "value": value, "unit": "quartets/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded random cluster, random PSD density)",
+           "config": {"workload": "FP64 J+K Fock build, %s, n=%d basis functions, %d shell quartets (8-fold)"
+                                  % (mol["name"], n, quartets),
+                      "n_basis": n, "quartets_per_build": quartets, "fock_build_s": ms / 1e3,
+                      "parallelism": "single GPU"},
+           "roofline": roof, "cpu_baseline": cpu,
+           "classes": [{k: r[k] for k in ("la", "lb", "lc", "ld", "wavefront_per_quartet", "quartets")} |
+                       {"ms": r["device_ms"] / max(1, r["launches"]),
+                        "tflops": r["flops"] / max(1e-12, r["device_ms"] / 1e3) / 1e12} for r in rep]}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

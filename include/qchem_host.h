@@ -1,0 +1,69 @@
+/*
+ * qchem_host.h -- C ABI of the C++20 host side (libqchem_host.so), for bindings and tools.
+ *
+ * The RHF driver here is the caller side of the engine boundary: it restates the reference's
+ * SCF recipe (Calculation::Converge, src/Calculation/Imp/Calculation.C:166-230: core guess, DIIS
+ * with FDMax = Z^2 0.1/32, SCFIterator::Iterate SCFIterator.C:240-409) and reaches J/K only through
+ * the Orbital_HF_IBS / WholeSystemFock_IBS faces implemented by the GpuHF_IBS adapter.
+ */
+#ifndef QCHEM_HOST_H
+#define QCHEM_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A molecular basis as raw shells.  contracted[s] != 0 selects the contracted-radial convention
+ * (coefficient * primitive norm of x^L); 0 the single-exponent radial (coefficient 1). */
+typedef struct qchost_shells {
+    int32_t nshell;
+    const int32_t* l;
+    const int32_t* nprim;
+    const double* exps;
+    const double* coefs;
+    const int32_t* contracted;
+    const double* centers; /* 3*nshell */
+} qchost_shells;
+
+typedef struct qcscf_params {
+    int32_t max_iter;
+    double e_tol;      /* |dE| convergence (Eh) */
+    double fd_tol;     /* ||[F',D']|| convergence */
+    int32_t nproj;     /* DIIS history */
+    double fd_max;     /* DIIS engage threshold; <= 0 selects Z^2 0.1/32 */
+    double sv_tol;     /* DIIS bordered-matrix svmin prune */
+    uint32_t jk_flags; /* QCJK_J|QCJK_K plus QCJK_SCREEN / QCJK_MIXED */
+    double screen_tol;
+    double mixed_tol;
+    int32_t device;
+    int32_t verbose;
+} qcscf_params;
+
+typedef struct qcscf_result {
+    double energy, e_nuc, e_one, e_coul, e_exch;
+    int32_t iterations;
+    int32_t converged;
+    int32_t nbf;
+    double t_total, t_jk, t_diag, t_setup;
+} qcscf_result;
+
+/* Folded coefficients and ns for the shells (the conventions the engine expects). */
+int qchost_basis_prepare(const qchost_shells* sh, double* folded /* sum nprim */, double* ns /* nbf */, int32_t* nbf);
+
+/* Closed-shell RHF.  D0 (nullable, nbf x nbf) seeds the SCF instead of the core guess.
+ * D_out (nullable) receives the final density, e_hist/t_hist (nullable, max_iter) the energy and
+ * elapsed seconds after every Fock build.  Returns a qcjk status code (or 100 + exception). */
+int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const double* R, int32_t charge,
+              const qcscf_params* par, const double* D0, qcscf_result* res, double* D_out, double* e_hist,
+              double* t_hist);
+
+/* Last error message of this thread (exception text), or "". */
+const char* qchost_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QCHEM_HOST_H */

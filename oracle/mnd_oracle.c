@@ -1,0 +1,760 @@
+/*
+ * mnd_oracle.c -- CPU ORACLE for the McMurchie-Davidson Fock-build path.
+ *
+ * THIS IS TEST INFRASTRUCTURE, NOT PRODUCT CODE.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as the checker (or the timed CPU baseline).
+ * The product path (qchem_amd/, libqcjk.so) never links, loads or calls anything here.
+ *
+ * It restates, step for step, the reference algorithm of JanReimers/qchem for this path
+ * (citations relative to the reference root):
+ *   Boys F_j(T)          src/BasisSet/Molecule/Evaluators/Internal/Boys/Imp/AuxillaryFJ.C:13-48
+ *                        (the 121x23 F_j(0.1k) table is REGENERATED here from the convergent series
+ *                         F_j(T) = e^-T sum_k (2T)^k / ((2j+1)(2j+3)...(2j+2k+1)), not copied)
+ *   Hermite2 E^{ij}_t    src/BasisSet/Molecule/Evaluators/PG_Cart_MnD/Internal/Imp/Hermite2.C:95-139
+ *   R_tuv recursion      src/BasisSet/Molecule/Evaluators/Internal/MnD/Imp/RNLM.C:19-68
+ *   Omega (pair) data    src/BasisSet/Molecule/Evaluators/PG_Cart_MnD/Imp/GaussianRF.C:239-252
+ *   prim (ab|cd)         .../PG_Cart_MnD/Imp/GaussianRF.C:446-483
+ *   contraction          .../PG_Cart_MnD/Imp/GaussianRF.C:726-752
+ *   FourC (ns folded)    .../PG_Cart_MnD/Evaluator.C:1912-1920
+ *   1E: overlap/grad2/nuclear  .../PG_Cart_MnD/Imp/GaussianRF.C:311-405
+ *   prim normalisation   .../PG_Cart_MnD/Imp/GaussianRF.C:509-521, 566-577
+ *   ns = 1/sqrt(S_ii)    .../PG_Cart_MnD/Imp/PGData.C:17-33
+ *   component order      src/BasisSet/Molecule/PG_Cart/Imp/IrrepBasisSet.C:46-54
+ *   ERI4 Direct/Exchange src/BasisSet/Molecule/IrrepBasisSet.C:138-183
+ *   ERI4::MatMul         src/BasisSet/Internal/Imp/ERI4.C:10-17
+ *
+ * Two J/K routes:
+ *   or_jk_refalgo  -- the reference algorithm: materialise the J4/K4 tensors element by element
+ *                     through FourC with an R_tuv cache keyed by (Omega_ab, Omega_cd), then MatMul.
+ *                     Single-threaded like the reference (its ERI path is serial).
+ *   or_jk_direct   -- same mathematics, integral-direct over unique shell quartets (8-fold
+ *                     symmetry), OpenMP over bra shell pairs.  Checked against or_jk_refalgo.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.14159265358979323846264338327950288
+#define OR_JMAX 16              /* RNLM.C:10  MAX 16 */
+#define OR_TAB_ROWS 121         /* AuxillaryFJ.C: theLookUp[121][23] */
+#define OR_TAB_COLS 23
+
+/* ------------------------------------------------------------------------------------------ */
+/* Boys function                                                                              */
+/* ------------------------------------------------------------------------------------------ */
+static double g_boys_tab[OR_TAB_ROWS][OR_TAB_COLS];
+static int g_boys_init = 0;
+
+/* Accurate F_j(T) by the positive-term series (long double); used to regenerate the table. */
+static long double boys_series(int j, long double T)
+{
+    long double term = 1.0L / (2 * j + 1), sum = term;
+    for (int k = 1; k < 400; ++k) {
+        term *= 2.0L * T / (2 * j + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-22L) break;
+    }
+    return expl(-T) * sum;
+}
+
+static void boys_init(void)
+{
+    if (g_boys_init) return;
+    for (int it = 0; it < OR_TAB_ROWS; ++it)
+        for (int j = 0; j < OR_TAB_COLS; ++j)
+            g_boys_tab[it][j] = (double)boys_series(j, 0.1L * it);
+    g_boys_init = 1;
+}
+
+/* Series value, exported so tests can pin the regenerated table against mpmath. */
+double or_boys_series(int j, double T) { return (double)boys_series(j, (long double)T); }
+
+/* Boys mode: 0 = the reference algorithm (default); 1 = F_jmax from the convergent series in long
+ * double plus downward recursion (used by tests to separate the reference's Taylor-step error,
+ * ~1e-12 relative, from the arithmetic under test). */
+static int g_boys_mode = 0;
+void or_set_boys_mode(int mode) { g_boys_mode = mode; }
+
+/* AuxillaryFJ::GetFjAt (AuxillaryFJ.C:13-48): Fj[0..jmax]. */
+void or_boys_ref(double T, int jmax, double* Fj)
+{
+    if (g_boys_mode == 1) {
+        const long double TL = (long double)T, eT = expl(-TL);
+        if (T < 30.0) {
+            long double F = boys_series(jmax, TL);
+            Fj[jmax] = (double)F;
+            for (int j = jmax - 1; j >= 0; j--) { F = (2.0L * TL * F + eT) / (2 * j + 1); Fj[j] = (double)F; }
+        } else {
+            long double F = 0.5L * sqrtl(3.14159265358979323846264338327950288L / TL) * erfl(sqrtl(TL));
+            Fj[0] = (double)F;
+            for (int j = 0; j < jmax; j++) { F = ((2 * j + 1) * F - eT) / (2.0L * TL); Fj[j + 1] = (double)F; }
+        }
+        return;
+    }
+    static const double fact[7] = {1, 1, 2, 6, 24, 120, 720};
+    boys_init();
+    double et = exp(-T);
+    if (T < 12) {
+        int it = (int)(T * 10.0 + 0.5);
+        double Tstar = (double)it * 0.1;
+        double dt = Tstar - T;
+        const double* FT = &g_boys_tab[it][0];
+        double fjmax = FT[jmax];
+        double dtk = 1.0;
+        for (int k = 1; k <= 6; k++) { dtk *= dt; fjmax += FT[k + jmax] * dtk / fact[k]; }
+        Fj[jmax] = fjmax;
+        for (int j = jmax - 1; j >= 0; j--) Fj[j] = (2 * T * Fj[j + 1] + et) / (2 * j + 1);
+    }
+    if (T >= 12 && T < 30.0) {
+        /* M&D 1978 fitted asymptote for F_0 (the reference's constants are the published fit). */
+        double g = 0;
+        if (T < 15) g = 0.4999489092 - 0.2473631686 / T + 0.321180909 / (T * T) - 0.3811559346 / (T * T * T);
+        if (T >= 15 && T < 18) g = 0.4998436875 - 0.24249438 / T + 0.24642845 / (T * T);
+        if (T >= 18 && T < 24) g = 0.499093162 - 0.2152832 / T;
+        if (T >= 24) g = 0.49;
+        Fj[0] = 0.5 * sqrt(OR_PI / T) - et * g / T;
+        for (int j = 0; j < jmax; j++) Fj[j + 1] = ((2 * j + 1) * Fj[j] - et) / (2 * T);
+    }
+    if (T >= 30.0) {
+        Fj[0] = 0.5 * sqrt(OR_PI / T);
+        for (int j = 0; j < jmax; j++) Fj[j + 1] = ((2 * j + 1) * Fj[j] - et) / (2 * T);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Hermite2: per-axis E^{na,nb}_N, N <= na+nb.  Dense layout e[(N*(LA+1)+na)*(LB+1)+nb].       */
+/* ------------------------------------------------------------------------------------------ */
+#define HMAX 4 /* la+1 <= 4 (f shells with the +1 used by the kinetic term) */
+typedef struct {
+    int LA, LB;                              /* table extents (la+1, lb+1 as in the Omega ctor) */
+    double e[3][2 * HMAX + 1][HMAX + 1][HMAX + 1];
+} or_h2;
+
+static void hermite2_build(or_h2* h, double p, const double PA[3], const double PB[3], int LA, int LB)
+{
+    h->LA = LA; h->LB = LB;
+    memset(h->e, 0, sizeof(h->e));
+    double a12 = 1.0 / (2 * p);
+    for (int x = 0; x < 3; ++x) h->e[x][0][0][0] = 1.0;
+    for (int layer = 0; layer <= LA + LB - 1; layer++)
+        for (int na = 0; na <= layer; na++) {
+            int nb = layer - na;
+            if (na > LA || nb > LB) continue;
+            for (int N = 0; N <= layer + 1; N++) {
+                for (int x = 0; x < 3; ++x) {
+                    double t1 = 0, ta = 0, tb = 0, t3 = 0;
+                    if (N - 1 >= 0) t1 = h->e[x][N - 1][na][nb] * a12;
+                    if (N <= na + nb) { ta = h->e[x][N][na][nb] * PA[x]; tb = h->e[x][N][na][nb] * PB[x]; }
+                    if (N + 1 <= na + nb) t3 = h->e[x][N + 1][na][nb] * (N + 1);
+                    if (na + 1 <= LA) h->e[x][N][na + 1][nb] = t1 + ta + t3;
+                    if (nb + 1 <= LB) h->e[x][N][na][nb + 1] = t1 + tb + t3;
+                }
+            }
+        }
+}
+
+/* Hermite2::operator() with the negative-index guard (Hermite2.C:158-199). */
+static inline double h2_get(const or_h2* h, const int P[3], const int a[3], const int b[3])
+{
+    for (int x = 0; x < 3; ++x) if (P[x] < 0 || a[x] < 0 || b[x] < 0) return 0.0;
+    return h->e[0][P[0]][a[0]][b[0]] * h->e[1][P[1]][a[1]][b[1]] * h->e[2][P[2]][a[2]][b[2]];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* RNLM: R^{(0)}_{tuv} for t+u+v <= L, stored dense r[(t*(L+1)+u)*(L+1)+v].                   */
+/* ------------------------------------------------------------------------------------------ */
+static void rnlm_build(int L, double alpha, const double dR[3], double* out /* (L+1)^3 */)
+{
+    int n1 = L + 1;
+    double Fj[OR_JMAX + 1];
+    double T = alpha * (dR[0] * dR[0] + dR[1] * dR[1] + dR[2] * dR[2]);
+    or_boys_ref(T, L, Fj);
+    /* level buffers: cur = level j, nxt = level j+1 */
+    size_t sz = (size_t)n1 * n1 * n1;
+    double* lev = (double*)calloc(2 * sz, sizeof(double));
+    double* nxt = lev;      /* level j+1 */
+    double* cur = lev + sz; /* level j   */
+#define IX(t, u, v) (((t) * n1 + (u)) * n1 + (v))
+    double m2a = -2.0 * alpha;
+    double pw = 1.0;
+    for (int j = 0; j < L; ++j) pw *= m2a;
+    nxt[IX(0, 0, 0)] = pw * Fj[L];
+    for (int j = L - 1; j >= 0; j--) {
+        pw = 1.0;
+        for (int k = 0; k < j; ++k) pw *= m2a;
+        memset(cur, 0, sz * sizeof(double));
+        cur[IX(0, 0, 0)] = pw * Fj[j];
+        int n = L - j;
+        for (int M = 0; M < n; M++) {
+            cur[IX(0, 0, M + 1)] = M > 0 ? dR[2] * nxt[IX(0, 0, M)] + M * nxt[IX(0, 0, M - 1)] : dR[2] * nxt[IX(0, 0, M)];
+            for (int Lq = 0; Lq < n - M; Lq++) {
+                cur[IX(0, Lq + 1, M)] = Lq > 0 ? dR[1] * nxt[IX(0, Lq, M)] + Lq * nxt[IX(0, Lq - 1, M)] : dR[1] * nxt[IX(0, Lq, M)];
+                for (int N = 0; N < n - M - Lq; N++)
+                    cur[IX(N + 1, Lq, M)] = N > 0 ? dR[0] * nxt[IX(N, Lq, M)] + N * nxt[IX(N - 1, Lq, M)] : dR[0] * nxt[IX(N, Lq, M)];
+            }
+        }
+        double* t = nxt; nxt = cur; cur = t;
+    }
+    memcpy(out, nxt, sz * sizeof(double));
+    free(lev);
+#undef IX
+}
+
+/* Exported for tests: R^{(0)}_{tuv} dense cube. */
+void or_rnlm(int L, double alpha, const double* dR, double* out) { rnlm_build(L, alpha, dR, out); }
+
+/* Exported for tests: per-axis E^{na,nb}_N into out[x][N][na][nb] dense with extents (la+lb+1,la+1,lb+1). */
+void or_hermite2(double p, const double* PA, const double* PB, int la, int lb, double* out)
+{
+    or_h2 h;
+    hermite2_build(&h, p, PA, PB, la, lb);
+    int nN = la + lb + 1;
+    for (int x = 0; x < 3; ++x)
+        for (int N = 0; N < nN; ++N)
+            for (int a = 0; a <= la; ++a)
+                for (int b = 0; b <= lb; ++b)
+                    out[((x * nN + N) * (la + 1) + a) * (lb + 1) + b] = h.e[x][N][a][b];
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Basis                                                                                      */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int nshell, nbf, nprim_tot;
+    int *L, *nprim, *poff, *bfoff;
+    double *cen, *exps, *coef;
+    int *fshell, *pol; /* per function: shell, (n,l,m) */
+    double* ns;
+} or_basis;
+
+static int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+/* component k of shell L in reference order: m outer, l inner, n = L-m-l */
+static void cart_comp(int L, int k, int out[3])
+{
+    int c = 0;
+    for (int m = 0; m <= L; m++)
+        for (int l = 0; l <= L - m; l++) {
+            if (c == k) { out[0] = L - m - l; out[1] = l; out[2] = m; return; }
+            c++;
+        }
+}
+void or_cart_comp(int L, int k, int* out) { cart_comp(L, k, out); }
+
+static const double g_dfact[14] = {1, 1, 1, 2, 3, 8, 15, 48, 105, 384, 945, 3840, 10395, 46080};
+static double dfact(int i) { return g_dfact[i + 1]; }
+
+/* PrimGaussian::GetNormalization for polarization (L,0,0) (GaussianRF.C:512-521) */
+static double prim_norm_L00(double a, int L)
+{
+    double s = pow(OR_PI / (2 * a), 1.5);
+    double t = 1.0;
+    for (int i = 0; i < L; ++i) t *= 4 * a;
+    double f = dfact(2 * L - 1) * dfact(-1) * dfact(-1);
+    return sqrt(t / (s * f));
+}
+
+/* Omega for a primitive pair (GaussianRF.C:239-252). Hermite tables sized (la+1, lb+1). */
+typedef struct {
+    double a, b, p, P[3], Eij, AB[3];
+    int la, lb;
+    or_h2 h;
+} or_omega;
+
+static void omega_build(or_omega* w, double a, const double* A, int la, double b, const double* B, int lb)
+{
+    w->a = a; w->b = b; w->p = a + b; w->la = la; w->lb = lb;
+    double r2 = 0, PA[3], PB[3];
+    for (int x = 0; x < 3; ++x) {
+        w->AB[x] = A[x] - B[x];
+        w->P[x] = (a * A[x] + b * B[x]) / w->p;
+        r2 += w->AB[x] * w->AB[x];
+        PA[x] = w->P[x] - A[x];
+        PB[x] = w->P[x] - B[x];
+    }
+    w->Eij = exp(-a * b / w->p * r2);
+    hermite2_build(&w->h, w->p, PA, PB, la + 1, lb + 1);
+}
+
+static double prim_overlap(const or_omega* w, const int pa[3], const int pb[3])
+{
+    int z[3] = {0, 0, 0};
+    return pow(OR_PI / w->p, 1.5) * w->Eij * h2_get(&w->h, z, pa, pb);
+}
+
+/* GetGrad2 (GaussianRF.C:311-333): <-nabla^2> building block */
+static double prim_grad2(const or_omega* w, const int pa[3], const int pb[3])
+{
+    int z[3] = {0, 0, 0};
+    double h = 0;
+    for (int x = 0; x < 3; ++x) {
+        int am[3] = {pa[0], pa[1], pa[2]}, ap[3] = {pa[0], pa[1], pa[2]};
+        int bm[3] = {pb[0], pb[1], pb[2]}, bp[3] = {pb[0], pb[1], pb[2]};
+        am[x]--; ap[x]++; bm[x]--; bp[x]++;
+        h += pa[x] * pb[x] * h2_get(&w->h, z, am, bm)
+           - 2 * pa[x] * w->b * h2_get(&w->h, z, am, bp)
+           - 2 * pb[x] * w->a * h2_get(&w->h, z, ap, bm)
+           + 4 * w->a * w->b * h2_get(&w->h, z, ap, bp);
+    }
+    double factor = pow(OR_PI / w->p, 1.5) * w->Eij;
+    return h != 0 ? factor * h : 0.0;
+}
+
+/* Nuclear (GaussianRF.C:386-405): sum over atoms of -Z R, contracted with E. */
+static double prim_nuclear(const or_omega* w, const int pa[3], const int pb[3], int natom, const double* Z, const double* C)
+{
+    int L = w->la + w->lb, n1 = L + 1;
+    size_t sz = (size_t)n1 * n1 * n1;
+    double* R = (double*)calloc(sz, sizeof(double));
+    double* Ra = (double*)malloc(sz * sizeof(double));
+    for (int i = 0; i < natom; ++i) {
+        double dR[3] = {w->P[0] - C[3 * i], w->P[1] - C[3 * i + 1], w->P[2] - C[3 * i + 2]};
+        rnlm_build(L, w->p, dR, Ra);
+        for (size_t k = 0; k < sz; ++k) R[k] += -Z[i] * Ra[k];
+    }
+    double s = 0;
+    int Pab[3] = {pa[0] + pb[0], pa[1] + pb[1], pa[2] + pb[2]};
+    for (int t = 0; t <= Pab[0]; ++t)
+        for (int u = 0; u <= Pab[1]; ++u)
+            for (int v = 0; v <= Pab[2]; ++v) {
+                int N[3] = {t, u, v};
+                double h = h2_get(&w->h, N, pa, pb);
+                if (h != 0) s += h * R[(t * n1 + u) * n1 + v];
+            }
+    free(R); free(Ra);
+    return s * 2 * OR_PI / w->p * w->Eij;
+}
+
+/* PrimGaussian::Repulsion4C (GaussianRF.C:446-483) with R supplied (dense cube, extent nR). */
+static double prim_eri_R(const or_omega* ab, const or_omega* cd, const double* R, int nR,
+                         const int pa[3], const int pb[3], const int pc[3], const int pd[3])
+{
+    double lambda = 2 * pow(OR_PI, 2.5) / (ab->p * cd->p * sqrt(ab->p + cd->p));
+    lambda *= ab->Eij * cd->Eij;
+    int Pab[3] = {pa[0] + pb[0], pa[1] + pb[1], pa[2] + pb[2]};
+    int Pcd[3] = {pc[0] + pd[0], pc[1] + pd[1], pc[2] + pd[2]};
+    double s = 0.0;
+    for (int an = 0; an <= Pab[0]; ++an)
+        for (int al = 0; al <= Pab[1]; ++al)
+            for (int am = 0; am <= Pab[2]; ++am) {
+                int A3[3] = {an, al, am};
+                double hab = h2_get(&ab->h, A3, pa, pb);
+                if (hab == 0.0) continue;
+                for (int cn = 0; cn <= Pcd[0]; ++cn)
+                    for (int cl = 0; cl <= Pcd[1]; ++cl)
+                        for (int cm = 0; cm <= Pcd[2]; ++cm) {
+                            int C3[3] = {cn, cl, cm};
+                            double hcd = h2_get(&cd->h, C3, pc, pd);
+                            if (hcd == 0.0) continue;
+                            double r = R[((an + cn) * nR + (al + cl)) * nR + (am + cm)];
+                            if (r != 0.0) {
+                                double sign = ((cn + cl + cm) & 1) ? -1.0 : 1.0;
+                                s += hab * hcd * r * sign;
+                            }
+                        }
+            }
+    return s * lambda;
+}
+
+static void omega_rnlm(const or_omega* ab, const or_omega* cd, double* R /* (L+1)^3 */)
+{
+    double alpha = ab->p * cd->p / (ab->p + cd->p);
+    double dR[3] = {ab->P[0] - cd->P[0], ab->P[1] - cd->P[1], ab->P[2] - cd->P[2]};
+    rnlm_build(ab->la + ab->lb + cd->la + cd->lb, alpha, dR, R);
+}
+
+/* Contracted radial-pair overlap for one component pair (GaussianRF::Overlap2C) */
+static double contracted_overlap(const or_basis* B, int sa, int sb, const int pa[3], const int pb[3])
+{
+    double s = 0;
+    for (int i = 0; i < B->nprim[sa]; ++i)
+        for (int j = 0; j < B->nprim[sb]; ++j) {
+            or_omega w;
+            omega_build(&w, B->exps[B->poff[sa] + i], &B->cen[3 * sa], B->L[sa],
+                        B->exps[B->poff[sb] + j], &B->cen[3 * sb], B->L[sb]);
+            s += B->coef[B->poff[sa] + i] * B->coef[B->poff[sb] + j] * prim_overlap(&w, pa, pb);
+        }
+    return s;
+}
+
+/*
+ * Build a basis.  Shell s has angular momentum L[s], nprim[s] primitives (exps/coefs consumed in
+ * shell order) at cen[3s..].  fold[s]!=0 selects the contracted constructor (coefficient times the
+ * primitive norm of x^L, GaussianRF.C:566-577); fold[s]==0 the single-exponent constructor
+ * (coefficient 1, GaussianRF.C:557-564).  Functions are flattened shell by shell, components in
+ * reference order, and ns = 1/sqrt(S_ii) (PGData.C:27-31).
+ */
+void* or_basis_create(int nshell, const int* L, const int* nprim, const double* exps, const double* coefs,
+                      const int* fold, const double* cen)
+{
+    or_basis* B = (or_basis*)calloc(1, sizeof(or_basis));
+    B->nshell = nshell;
+    B->L = (int*)malloc(nshell * sizeof(int));
+    B->nprim = (int*)malloc(nshell * sizeof(int));
+    B->poff = (int*)malloc(nshell * sizeof(int));
+    B->bfoff = (int*)malloc(nshell * sizeof(int));
+    B->cen = (double*)malloc(3 * nshell * sizeof(double));
+    int np = 0, nbf = 0;
+    for (int s = 0; s < nshell; ++s) {
+        B->L[s] = L[s]; B->nprim[s] = nprim[s]; B->poff[s] = np; B->bfoff[s] = nbf;
+        np += nprim[s]; nbf += ncart(L[s]);
+        for (int x = 0; x < 3; ++x) B->cen[3 * s + x] = cen[3 * s + x];
+    }
+    B->nprim_tot = np; B->nbf = nbf;
+    B->exps = (double*)malloc(np * sizeof(double));
+    B->coef = (double*)malloc(np * sizeof(double));
+    for (int s = 0; s < nshell; ++s)
+        for (int i = 0; i < nprim[s]; ++i) {
+            int k = B->poff[s] + i;
+            B->exps[k] = exps[k];
+            B->coef[k] = fold[s] ? coefs[k] * prim_norm_L00(exps[k], L[s]) : 1.0;
+        }
+    B->fshell = (int*)malloc(nbf * sizeof(int));
+    B->pol = (int*)malloc(3 * nbf * sizeof(int));
+    B->ns = (double*)malloc(nbf * sizeof(double));
+    for (int s = 0; s < nshell; ++s)
+        for (int k = 0; k < ncart(L[s]); ++k) {
+            int f = B->bfoff[s] + k;
+            B->fshell[f] = s;
+            cart_comp(L[s], k, &B->pol[3 * f]);
+            double Sii = contracted_overlap(B, s, s, &B->pol[3 * f], &B->pol[3 * f]);
+            B->ns[f] = 1.0 / sqrt(Sii);
+        }
+    return B;
+}
+
+void or_basis_destroy(void* h)
+{
+    or_basis* B = (or_basis*)h;
+    if (!B) return;
+    free(B->L); free(B->nprim); free(B->poff); free(B->bfoff); free(B->cen);
+    free(B->exps); free(B->coef); free(B->fshell); free(B->pol); free(B->ns);
+    free(B);
+}
+
+int or_basis_nbf(void* h) { return ((or_basis*)h)->nbf; }
+void or_basis_ns(void* h, double* out) { or_basis* B = (or_basis*)h; memcpy(out, B->ns, B->nbf * sizeof(double)); }
+void or_basis_coef(void* h, double* out) { or_basis* B = (or_basis*)h; memcpy(out, B->coef, B->nprim_tot * sizeof(double)); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* One-electron matrices: S, T = 0.5 <-nabla^2>, V (Hamiltonians.C:36-43, Kinetic.C:57)       */
+/* ------------------------------------------------------------------------------------------ */
+void or_one_electron(void* h, int natom, const double* Z, const double* C, double* S, double* T, double* V)
+{
+    or_basis* B = (or_basis*)h;
+    int n = B->nbf;
+#pragma omp parallel for schedule(dynamic)
+    for (int f = 0; f < n; ++f)
+        for (int g = 0; g < n; ++g) {
+            int sa = B->fshell[f], sb = B->fshell[g];
+            const int *pa = &B->pol[3 * f], *pb = &B->pol[3 * g];
+            double s = 0, t = 0, v = 0;
+            for (int i = 0; i < B->nprim[sa]; ++i)
+                for (int j = 0; j < B->nprim[sb]; ++j) {
+                    or_omega w;
+                    omega_build(&w, B->exps[B->poff[sa] + i], &B->cen[3 * sa], B->L[sa],
+                                B->exps[B->poff[sb] + j], &B->cen[3 * sb], B->L[sb]);
+                    double cc = B->coef[B->poff[sa] + i] * B->coef[B->poff[sb] + j];
+                    s += cc * prim_overlap(&w, pa, pb);
+                    t += cc * prim_grad2(&w, pa, pb);
+                    v += cc * prim_nuclear(&w, pa, pb, natom, Z, C);
+                }
+            double nn = B->ns[f] * B->ns[g];
+            S[f * n + g] = s * nn;
+            T[f * n + g] = 0.5 * t * nn;
+            V[f * n + g] = v * nn;
+        }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Contracted shell-quartet block (all components, ns folded), direct evaluation               */
+/* out[((a*nb+b)*nc+c)*nd+d]                                                                   */
+/* ------------------------------------------------------------------------------------------ */
+static void shell_quartet(const or_basis* B, int sA, int sB, int sC, int sD, double* out)
+{
+    int la = B->L[sA], lb = B->L[sB], lc = B->L[sC], ld = B->L[sD];
+    int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    int L = la + lb + lc + ld, nR = L + 1;
+    int nout = na * nb * nc * nd;
+    int pols[4][6][3];
+    for (int k = 0; k < na; ++k) cart_comp(la, k, pols[0][k]);
+    for (int k = 0; k < nb; ++k) cart_comp(lb, k, pols[1][k]);
+    for (int k = 0; k < nc; ++k) cart_comp(lc, k, pols[2][k]);
+    for (int k = 0; k < nd; ++k) cart_comp(ld, k, pols[3][k]);
+    memset(out, 0, nout * sizeof(double));
+    double* R = (double*)malloc((size_t)nR * nR * nR * sizeof(double));
+    int nkl = B->nprim[sC] * B->nprim[sD];
+    or_omega* wcd = (or_omega*)malloc(nkl * sizeof(or_omega));
+    for (int k = 0; k < B->nprim[sC]; ++k)
+        for (int l = 0; l < B->nprim[sD]; ++l)
+            omega_build(&wcd[k * B->nprim[sD] + l], B->exps[B->poff[sC] + k], &B->cen[3 * sC], lc,
+                        B->exps[B->poff[sD] + l], &B->cen[3 * sD], ld);
+    for (int i = 0; i < B->nprim[sA]; ++i)
+        for (int j = 0; j < B->nprim[sB]; ++j) {
+            or_omega wab;
+            omega_build(&wab, B->exps[B->poff[sA] + i], &B->cen[3 * sA], la, B->exps[B->poff[sB] + j], &B->cen[3 * sB], lb);
+            double cij = B->coef[B->poff[sA] + i] * B->coef[B->poff[sB] + j];
+            for (int k = 0; k < B->nprim[sC]; ++k)
+                for (int l = 0; l < B->nprim[sD]; ++l) {
+                    const or_omega* w2 = &wcd[k * B->nprim[sD] + l];
+                    double c = cij * B->coef[B->poff[sC] + k] * B->coef[B->poff[sD] + l];
+                    omega_rnlm(&wab, w2, R);
+                    int o = 0;
+                    for (int a = 0; a < na; ++a)
+                        for (int b = 0; b < nb; ++b)
+                            for (int cc = 0; cc < nc; ++cc)
+                                for (int d = 0; d < nd; ++d, ++o)
+                                    out[o] += c * prim_eri_R(&wab, w2, R, nR, pols[0][a], pols[1][b], pols[2][cc], pols[3][d]);
+                }
+        }
+    int o = 0;
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b)
+            for (int cc = 0; cc < nc; ++cc)
+                for (int d = 0; d < nd; ++d, ++o)
+                    out[o] *= B->ns[B->bfoff[sA] + a] * B->ns[B->bfoff[sB] + b] * B->ns[B->bfoff[sC] + cc] * B->ns[B->bfoff[sD] + d];
+    free(R); free(wcd);
+}
+
+void or_shell_quartet(void* h, int sA, int sB, int sC, int sD, double* out) { shell_quartet((or_basis*)h, sA, sB, sC, sD, out); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Reference-algorithm J/K: ERI4 materialisation through FourC + R cache, then MatMul          */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+    const or_basis* B;
+    int nomega;
+    or_omega** omega;  /* per ordered primitive pair (global prim index pair) */
+    double** rcache;   /* per (omega_ab, omega_cd) */
+    int* romega_L;
+    size_t rbytes;
+} or_refctx;
+
+static int prim_shell(const or_basis* B, int gp)
+{
+    int s = 0;
+    while (s + 1 < B->nshell && B->poff[s + 1] <= gp) ++s;
+    return s;
+}
+
+static const or_omega* find_omega(or_refctx* X, int gpa, int gpb)
+{
+    int id = gpa * X->B->nprim_tot + gpb;
+    if (!X->omega[id]) {
+        const or_basis* B = X->B;
+        int sa = prim_shell(B, gpa), sb = prim_shell(B, gpb);
+        or_omega* w = (or_omega*)malloc(sizeof(or_omega));
+        omega_build(w, B->exps[gpa], &B->cen[3 * sa], B->L[sa], B->exps[gpb], &B->cen[3 * sb], B->L[sb]);
+        X->omega[id] = w;
+    }
+    return X->omega[id];
+}
+
+static const double* find_rnlm(or_refctx* X, int idab, int idcd, const or_omega* ab, const or_omega* cd)
+{
+    size_t key = (size_t)idab * X->nomega + idcd;
+    if (!X->rcache[key]) {
+        int L = ab->la + ab->lb + cd->la + cd->lb, n = L + 1;
+        double* R = (double*)malloc((size_t)n * n * n * sizeof(double));
+        omega_rnlm(ab, cd, R);
+        X->rcache[key] = R;
+        X->rbytes += (size_t)n * n * n * sizeof(double);
+    }
+    return X->rcache[key];
+}
+
+/* GaussianRF::Repulsion4C x ns (Evaluator.C:1912-1920; GaussianRF.C:726-752) */
+static double fourc(or_refctx* X, int fa, int fb, int fc, int fd)
+{
+    const or_basis* B = X->B;
+    int sa = B->fshell[fa], sb = B->fshell[fb], sc = B->fshell[fc], sd = B->fshell[fd];
+    const int *pa = &B->pol[3 * fa], *pb = &B->pol[3 * fb], *pc = &B->pol[3 * fc], *pd = &B->pol[3 * fd];
+    double s = 0.0;
+    for (int i = 0; i < B->nprim[sa]; ++i)
+        for (int j = 0; j < B->nprim[sb]; ++j) {
+            int gi = B->poff[sa] + i, gj = B->poff[sb] + j;
+            int idab = gi * B->nprim_tot + gj;
+            const or_omega* ab = find_omega(X, gi, gj);
+            double cij = B->coef[gi] * B->coef[gj];
+            for (int k = 0; k < B->nprim[sc]; ++k)
+                for (int l = 0; l < B->nprim[sd]; ++l) {
+                    int gk = B->poff[sc] + k, gl = B->poff[sd] + l;
+                    int idcd = gk * B->nprim_tot + gl;
+                    const or_omega* cd = find_omega(X, gk, gl);
+                    const double* R = find_rnlm(X, idab, idcd, ab, cd);
+                    int nR = ab->la + ab->lb + cd->la + cd->lb + 1;
+                    s += cij * B->coef[gk] * B->coef[gl] * prim_eri_R(ab, cd, R, nR, pa, pb, pc, pd);
+                }
+        }
+    return s * B->ns[fa] * B->ns[fb] * B->ns[fc] * B->ns[fd];
+}
+
+static size_t tri(int a, int b) { return a <= b ? (size_t)b * (b + 1) / 2 + a : (size_t)a * (a + 1) / 2 + b; }
+
+/*
+ * J_ab = sum_cd (ab|cd) D_cd and K_ac = sum_bd (ab|cd) D_bd exactly as the reference builds them:
+ * J4 over ia<=ib, ic<=id (IrrepBasisSet.C:138-157), K4 with the (ib,id) half-symmetrised packing
+ * (IrrepBasisSet.C:159-183), then ERI4::MatMul over the symmetric inner block (ERI4.C:10-17).
+ * Returns 0, or -1 if the R cache would not fit (nprim_tot above 200).  times[0..2] = J4 build,
+ * K4 build, both MatMuls (seconds).  J/K are full row-major n x n.
+ */
+int or_jk_refalgo(void* h, const double* D, double* J, double* K, double* times)
+{
+    or_basis* B = (or_basis*)h;
+    int n = B->nbf;
+    if (B->nprim_tot > 200) return -1;
+    or_refctx X;
+    X.B = B;
+    X.nomega = B->nprim_tot * B->nprim_tot;
+    X.omega = (or_omega**)calloc(X.nomega, sizeof(or_omega*));
+    X.rcache = (double**)calloc((size_t)X.nomega * X.nomega, sizeof(double*));
+    X.rbytes = 0;
+    size_t npair = (size_t)n * (n + 1) / 2;
+    double* J4 = (double*)malloc(npair * npair * sizeof(double));
+    double* K4 = (double*)calloc(npair * npair, sizeof(double));
+    struct timespec t0, t1, t2, t3;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int ia = 0; ia < n; ia++)
+        for (int ib = ia; ib < n; ib++) {
+            double* blk = J4 + tri(ia, ib) * npair;
+            for (int ic = 0; ic < n; ic++)
+                for (int id = ic; id < n; id++) blk[tri(ic, id)] = fourc(&X, ia, ib, ic, id);
+        }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    for (int ia = 0; ia < n; ia++)
+        for (int ib = 0; ib < n; ib++)
+            for (int ic = ia; ic < n; ic++) {
+                double* blk = K4 + tri(ia, ic) * npair;
+                for (int id = 0; id < n; id++) {
+                    double v = fourc(&X, ia, ib, ic, id);
+                    if (ib == id) blk[tri(ib, id)] = v;
+                    else blk[tri(ib, id)] += 0.5 * v;
+                }
+            }
+    clock_gettime(CLOCK_MONOTONIC, &t2);
+    memset(J, 0, (size_t)n * n * sizeof(double));
+    memset(K, 0, (size_t)n * n * sizeof(double));
+    for (int ia = 0; ia < n; ia++)
+        for (int ib = ia; ib < n; ib++) {
+            const double* jb = J4 + tri(ia, ib) * npair;
+            const double* kb = K4 + tri(ia, ib) * npair;
+            double sj = 0, sk = 0;
+            for (int c = 0; c < n; ++c)
+                for (int d = 0; d < n; ++d) {
+                    sj += jb[tri(c, d)] * D[c * n + d];
+                    sk += kb[tri(c, d)] * D[c * n + d];
+                }
+            J[ia * n + ib] = J[ib * n + ia] = sj;
+            K[ia * n + ib] = K[ib * n + ia] = sk;
+        }
+    clock_gettime(CLOCK_MONOTONIC, &t3);
+    if (times) {
+        times[0] = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+        times[1] = (t2.tv_sec - t1.tv_sec) + 1e-9 * (t2.tv_nsec - t1.tv_nsec);
+        times[2] = (t3.tv_sec - t2.tv_sec) + 1e-9 * (t3.tv_nsec - t2.tv_nsec);
+    }
+    for (size_t i = 0; i < (size_t)X.nomega * X.nomega; ++i) free(X.rcache[i]);
+    for (int i = 0; i < X.nomega; ++i) free(X.omega[i]);
+    free(X.rcache); free(X.omega); free(J4); free(K4);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Direct 8-fold J/K (same mathematics, OpenMP over bra shell pairs)                           */
+/* ------------------------------------------------------------------------------------------ */
+/*
+ * For each unique shell quartet (A>=B, C>=D, pair(AB)>=pair(CD)) with degeneracy scale
+ * s = (A==B?1/2:1)(C==D?1/2:1)(AB==CD?1/2:1):
+ *   Jt_ab += 2s sum_cd I D_cd,  Jt_cd += 2s sum_ab I D_ab,
+ *   Kt_ac += s sum_bd I D_bd,   Kt_bc += s sum_ad I D_ad,  Kt_ad += s sum_bc I D_bc,  Kt_bd += s sum_ac I D_ac,
+ * then J = Jt + Jt^T, K = Kt + Kt^T.  pair_limit < 0 means all bra pairs; otherwise only the first
+ * pair_limit bra pairs (in the (A,B) enumeration) are processed, for bounded timing samples.
+ * Returns the number of contracted shell quartets evaluated.
+ */
+long long or_jk_direct(void* h, const double* D, double* J, double* K, int nthreads, long long pair_limit)
+{
+    or_basis* B = (or_basis*)h;
+    int n = B->nbf, ns = B->nshell;
+    int npairs = ns * (ns + 1) / 2;
+    int* pA = (int*)malloc(npairs * sizeof(int));
+    int* pB = (int*)malloc(npairs * sizeof(int));
+    int q = 0;
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) { pA[q] = a; pB[q] = b; ++q; }
+    long long nbra = (pair_limit >= 0 && pair_limit < npairs) ? pair_limit : npairs;
+    memset(J, 0, (size_t)n * n * sizeof(double));
+    memset(K, 0, (size_t)n * n * sizeof(double));
+    long long count = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(+ : count)
+    {
+        double* Jt = (double*)calloc((size_t)n * n, sizeof(double));
+        double* Kt = (double*)calloc((size_t)n * n, sizeof(double));
+        double* I = (double*)malloc(6 * 6 * 6 * 6 * sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (long long P = 0; P < nbra; ++P) {
+            int A = pA[P], Bs = pB[P];
+            for (int Q = 0; Q <= P; ++Q) {
+                int C = pA[Q], Ds = pB[Q];
+                shell_quartet(B, A, Bs, C, Ds, I);
+                ++count;
+                double s = (A == Bs ? 0.5 : 1.0) * (C == Ds ? 0.5 : 1.0) * (P == Q ? 0.5 : 1.0);
+                int na = ncart(B->L[A]), nb = ncart(B->L[Bs]), nc = ncart(B->L[C]), nd = ncart(B->L[Ds]);
+                int oa = B->bfoff[A], ob = B->bfoff[Bs], oc = B->bfoff[C], od = B->bfoff[Ds];
+                int o = 0;
+                for (int a = 0; a < na; ++a)
+                    for (int b = 0; b < nb; ++b)
+                        for (int c = 0; c < nc; ++c)
+                            for (int d = 0; d < nd; ++d, ++o) {
+                                double v = I[o] * s;
+                                int ga = oa + a, gb = ob + b, gc = oc + c, gd = od + d;
+                                Jt[ga * n + gb] += 2 * v * D[gc * n + gd];
+                                Jt[gc * n + gd] += 2 * v * D[ga * n + gb];
+                                Kt[ga * n + gc] += v * D[gb * n + gd];
+                                Kt[gb * n + gc] += v * D[ga * n + gd];
+                                Kt[ga * n + gd] += v * D[gb * n + gc];
+                                Kt[gb * n + gd] += v * D[ga * n + gc];
+                            }
+            }
+        }
+#pragma omp critical
+        {
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    J[i * n + j] += Jt[i * n + j] + Jt[j * n + i];
+                    K[i * n + j] += Kt[i * n + j] + Kt[j * n + i];
+                }
+        }
+        free(Jt); free(Kt); free(I);
+    }
+    free(pA); free(pB);
+    return count;
+}
+
+/* Schwarz bound Q_AB = sqrt(max_ab |(ab|ab)|) per shell pair, for screening analysis in tests. */
+void or_schwarz(void* h, double* Q /* nshell*nshell */)
+{
+    or_basis* B = (or_basis*)h;
+    int ns = B->nshell;
+    double* I = (double*)malloc(6 * 6 * 6 * 6 * sizeof(double));
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) {
+            shell_quartet(B, a, b, a, b, I);
+            int na = ncart(B->L[a]), nb = ncart(B->L[b]);
+            double m = 0;
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j) {
+                    double v = fabs(I[((i * nb + j) * na + i) * nb + j]);
+                    if (v > m) m = v;
+                }
+            Q[a * ns + b] = Q[b * ns + a] = sqrt(m);
+        }
+    free(I);
+}

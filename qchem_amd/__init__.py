@@ -1,0 +1,288 @@
+"""qchem_amd -- MI355X-native Fock-build engine for qchem's molecular Gaussian path.
+
+Python face of the two native libraries built in-tree by ``qchem_amd.build``:
+
+* ``libqcjk.so``: the HIP engine behind the C ABI in ``include/qcjk.h``.  It computes
+  McMurchie-Davidson ERIs digested into J and K, plus the one-electron matrices.
+* ``libqchem_host.so``: the C++20 host side in ``include/qchem_host.h``, namely the basis
+  conventions and the RHF driver that reaches J/K only through the HF faces.
+
+There is no CPU fallback.  If the libraries are not built, every call raises.  Without a GPU,
+engine creation fails with ``QCJK_ERR_NODEVICE``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import systems  # noqa: F401  (re-exported generators)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_HOST = None
+
+QCJK_J, QCJK_K, QCJK_MIXED, QCJK_SCREEN = 1, 2, 4, 8
+_ERRORS = {0: "ok", 1: "invalid argument", 2: "unsupported basis or option", 3: "HIP runtime error",
+           4: "out of memory", 5: "no usable HIP device"}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, where, code, detail=""):
+        self.code = code
+        super().__init__("%s failed: %s%s" % (where, _ERRORS.get(code, "error %d" % code),
+                                              (" (%s)" % detail) if detail else ""))
+
+
+class _Basis(ctypes.Structure):
+    _fields_ = [("nshell", ctypes.c_int32), ("shell_l", ctypes.POINTER(ctypes.c_int32)),
+                ("shell_nprim", ctypes.POINTER(ctypes.c_int32)), ("shell_center", ctypes.POINTER(ctypes.c_double)),
+                ("prim_exp", ctypes.POINTER(ctypes.c_double)), ("prim_coef", ctypes.POINTER(ctypes.c_double)),
+                ("bf_norm", ctypes.POINTER(ctypes.c_double)), ("order", ctypes.c_int32)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("screen_tol", ctypes.c_double), ("mixed_tol", ctypes.c_double),
+                ("reserved", ctypes.c_int32 * 8)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("quartets_total", ctypes.c_int64), ("quartets_computed", ctypes.c_int64),
+                ("quartets_fp32", ctypes.c_int64), ("prim_quartets", ctypes.c_int64),
+                ("flops", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ClassStat(ctypes.Structure):
+    _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32), ("ld", ctypes.c_int32),
+                ("wavefront_per_quartet", ctypes.c_int32), ("launches", ctypes.c_int64), ("quartets", ctypes.c_int64),
+                ("prim_quartets", ctypes.c_int64), ("flops", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class _Shells(ctypes.Structure):
+    _fields_ = [("nshell", ctypes.c_int32), ("l", ctypes.POINTER(ctypes.c_int32)),
+                ("nprim", ctypes.POINTER(ctypes.c_int32)), ("exps", ctypes.POINTER(ctypes.c_double)),
+                ("coefs", ctypes.POINTER(ctypes.c_double)), ("contracted", ctypes.POINTER(ctypes.c_int32)),
+                ("centers", ctypes.POINTER(ctypes.c_double))]
+
+
+class _ScfParams(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("e_tol", ctypes.c_double), ("fd_tol", ctypes.c_double),
+                ("nproj", ctypes.c_int32), ("fd_max", ctypes.c_double), ("sv_tol", ctypes.c_double),
+                ("jk_flags", ctypes.c_uint32), ("screen_tol", ctypes.c_double), ("mixed_tol", ctypes.c_double),
+                ("device", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+
+
+class _ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("e_nuc", ctypes.c_double), ("e_one", ctypes.c_double),
+                ("e_coul", ctypes.c_double), ("e_exch", ctypes.c_double), ("iterations", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("nbf", ctypes.c_int32), ("t_total", ctypes.c_double),
+                ("t_jk", ctypes.c_double), ("t_diag", ctypes.c_double), ("t_setup", ctypes.c_double)]
+
+
+QCJK_SYMBOLS = ["qcjk_create", "qcjk_nbf", "qcjk_build", "qcjk_build_device", "qcjk_one_electron",
+                "qcjk_schwarz", "qcjk_profile", "qcjk_class_report", "qcjk_destroy", "qcjk_strerror"]
+HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qchost_last_error"]
+
+
+def lib_paths():
+    return os.path.join(_HERE, "libqcjk.so"), os.path.join(_HERE, "libqchem_host.so")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def lib():
+    """The engine library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        path = lib_paths()[0]
+        if not os.path.exists(path):
+            raise RuntimeError("qchem_amd: %s is not built (run __graft_entry__.build())" % path)
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        L.qcjk_create.argtypes = [ctypes.POINTER(_Basis), ctypes.POINTER(_Opts), ctypes.POINTER(vp)]
+        L.qcjk_nbf.argtypes = [vp]
+        L.qcjk_build.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double), ctypes.c_uint, ctypes.POINTER(Stats)]
+        L.qcjk_build_device.argtypes = [vp, vp, vp, vp, ctypes.c_uint, vp, ctypes.POINTER(Stats)]
+        L.qcjk_one_electron.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.qcjk_schwarz.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.qcjk_profile.argtypes = [vp, ctypes.c_int]
+        L.qcjk_class_report.argtypes = [vp, ctypes.POINTER(ClassStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.qcjk_destroy.argtypes = [vp]
+        L.qcjk_strerror.argtypes = [ctypes.c_int]
+        L.qcjk_strerror.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def host_lib():
+    global _HOST
+    if _HOST is None:
+        lib()
+        path = lib_paths()[1]
+        if not os.path.exists(path):
+            raise RuntimeError("qchem_amd: %s is not built (run __graft_entry__.build())" % path)
+        H = ctypes.CDLL(path)
+        H.qchost_basis_prepare.argtypes = [ctypes.POINTER(_Shells), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
+        H.qcscf_rhf.argtypes = [ctypes.POINTER(_Shells), ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double), ctypes.c_int32, ctypes.POINTER(_ScfParams),
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ScfResult),
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]
+        H.qchost_last_error.restype = ctypes.c_char_p
+        _HOST = H
+    return _HOST
+
+
+class Basis:
+    """Molecular Cartesian basis: a list of shells ``dict(L, exps, coefs, center, contracted)``.
+
+    ``contracted`` selects the reference's contracted-radial convention (coefficients times the
+    primitive norm of x^L, GaussianRF.C:566-577); otherwise the single-exponent radial (coefficient 1).
+    Folded coefficients and the per-component ns come from the C++ host library.
+    """
+
+    def __init__(self, shells):
+        self.shells = [dict(s) for s in shells]
+        for s in self.shells:
+            s.setdefault("contracted", len(s["exps"]) > 1)
+        self.l = np.array([s["L"] for s in self.shells], dtype=np.int32)
+        self.nprim = np.array([len(s["exps"]) for s in self.shells], dtype=np.int32)
+        self.exps = np.array([e for s in self.shells for e in s["exps"]], dtype=np.float64)
+        self.coefs = np.array([c for s in self.shells for c in s["coefs"]], dtype=np.float64)
+        self.contracted = np.array([1 if s["contracted"] else 0 for s in self.shells], dtype=np.int32)
+        self.centers = np.ascontiguousarray(np.array([s["center"] for s in self.shells], dtype=np.float64).reshape(-1))
+        self.nbf = int(sum((s["L"] + 1) * (s["L"] + 2) // 2 for s in self.shells))
+        self.folded = np.zeros(len(self.exps))
+        self.ns = np.zeros(self.nbf)
+        nb = ctypes.c_int32(0)
+        rc = host_lib().qchost_basis_prepare(ctypes.byref(self._shells()), _dp(self.folded), _dp(self.ns), ctypes.byref(nb))
+        if rc != 0:
+            raise EngineError("qchost_basis_prepare", rc, host_lib().qchost_last_error().decode())
+
+    def _shells(self):
+        return _Shells(len(self.shells), _ip(self.l), _ip(self.nprim), _dp(self.exps), _dp(self.coefs),
+                       _ip(self.contracted), _dp(self.centers))
+
+    def _cbasis(self):
+        return _Basis(len(self.shells), _ip(self.l), _ip(self.nprim), _dp(self.centers), _dp(self.exps),
+                      _dp(self.folded), _dp(self.ns), 0)
+
+
+class JKEngine:
+    """One engine handle (one HIP stream).  ``build`` overwrites J and K for a symmetric D."""
+
+    def __init__(self, basis, device=0, screen_tol=0.0, mixed_tol=0.0):
+        self.basis = basis
+        self.n = basis.nbf
+        opts = _Opts(device, float(screen_tol), float(mixed_tol))
+        h = ctypes.c_void_p()
+        rc = lib().qcjk_create(ctypes.byref(basis._cbasis()), ctypes.byref(opts), ctypes.byref(h))
+        if rc != 0:
+            raise EngineError("qcjk_create", rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qcjk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, D, flags=QCJK_J | QCJK_K):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        assert D.shape == (self.n, self.n)
+        J = np.zeros_like(D)
+        K = np.zeros_like(D)
+        st = Stats()
+        rc = lib().qcjk_build(self.h, _dp(D), _dp(J), _dp(K), flags, ctypes.byref(st))
+        if rc != 0:
+            raise EngineError("qcjk_build", rc)
+        return J, K, st
+
+    def build_device(self, dD, dJ, dK, flags=QCJK_J | QCJK_K, stream=None, stats=False):
+        """Device pointers (ints) or torch CUDA tensors; asynchronous on ``stream`` unless stats."""
+        def ptr(x):
+            return None if x is None else (x if isinstance(x, int) else x.data_ptr())
+        st = Stats() if stats else None
+        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
+        rc = lib().qcjk_build_device(self.h, ptr(dD), ptr(dJ), ptr(dK), flags, s,
+                                     ctypes.byref(st) if st is not None else None)
+        if rc != 0:
+            raise EngineError("qcjk_build_device", rc)
+        return st
+
+    def one_electron(self, Z, R):
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+        n = self.n
+        S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+        rc = lib().qcjk_one_electron(self.h, len(Z), _dp(Z), _dp(R), _dp(S), _dp(T), _dp(V))
+        if rc != 0:
+            raise EngineError("qcjk_one_electron", rc)
+        return S, T, V
+
+    def profile(self, enable=True):
+        rc = lib().qcjk_profile(self.h, 1 if enable else 0)
+        if rc != 0:
+            raise EngineError("qcjk_profile", rc)
+
+    def class_report(self):
+        cnt = ctypes.c_int(0)
+        arr = (ClassStat * 64)()
+        rc = lib().qcjk_class_report(self.h, arr, 64, ctypes.byref(cnt))
+        if rc != 0:
+            raise EngineError("qcjk_class_report", rc)
+        return [arr[i].as_dict() for i in range(min(cnt.value, 64))]
+
+    def schwarz(self):
+        ns = len(self.basis.shells)
+        Q = np.zeros((ns, ns))
+        rc = lib().qcjk_schwarz(self.h, _dp(Q))
+        if rc != 0:
+            raise EngineError("qcjk_schwarz", rc)
+        return Q
+
+
+def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_max=-1.0, sv_tol=5e-9,
+        jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, D0=None, verbose=False):
+    """Closed-shell RHF through the C++ driver; returns a dict with energy, density and history."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+    par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device,
+                     1 if verbose else 0)
+    res = _ScfResult()
+    n = basis.nbf
+    Dout = np.zeros((n, n))
+    eh = np.zeros(max_iter)
+    th = np.zeros(max_iter)
+    d0 = None
+    if D0 is not None:
+        d0 = np.ascontiguousarray(D0, dtype=np.float64)
+    rc = host_lib().qcscf_rhf(ctypes.byref(basis._shells()), len(Z), _dp(Z), _dp(R), int(charge), ctypes.byref(par),
+                              _dp(d0) if d0 is not None else None, ctypes.byref(res), _dp(Dout), _dp(eh), _dp(th))
+    if rc != 0:
+        raise EngineError("qcscf_rhf", rc, host_lib().qchost_last_error().decode())
+    it = res.iterations
+    return {"energy": res.energy, "e_nuc": res.e_nuc, "e_one": res.e_one, "e_coul": res.e_coul,
+            "e_exch": res.e_exch, "iterations": it, "converged": bool(res.converged), "D": Dout,
+            "energy_history": eh[:it].copy(), "time_history": th[:it].copy(), "t_total": res.t_total,
+            "t_jk": res.t_jk, "t_diag": res.t_diag, "t_setup": res.t_setup}

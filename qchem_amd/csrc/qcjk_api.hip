@@ -1,0 +1,640 @@
+// qcjk_api.hip -- the C ABI declared in include/qcjk.h: handle life cycle, work lists, launches.
+//
+// Host-side responsibilities:
+//   * validate the flat basis, regenerate the Boys table (positive-term series in long double),
+//   * enumerate unique shell pairs (first shell has the higher angular momentum), group them by
+//     pair class, run the pair-precompute and Schwarz kernels, sort each class by the bound,
+//   * build, per (bra class >= ket class), the work-unit prefix sums the kernels binary-search,
+//   * per build: prescale D' = ns (x) ns o D, zero the accumulators, launch the 21 class kernels,
+//     symmetrise J = Jt + Jt^T (and K) with the ns post-scale.
+// No CPU fallback exists: every path needs the HIP device.
+#include "qcjk.h"
+#include "qcjk_internal.h"
+#include "mnd_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+using namespace qcjk;
+
+namespace {
+
+constexpr double kPruneTol = 1e-30;  // create-time: drop quartets whose Schwarz product is below this
+
+struct ClassPairWork {
+    int bra, ket;
+    int tile;
+    const ClassKernelInfo* info;
+    long long nunits = 0;
+    long long nquartets = 0;   // surviving at create time
+    long long* d_start = nullptr;
+    int* d_nsurv = nullptr;
+    double prim_flops = 0;     // per primitive quartet
+    double digest_flops = 0;   // per contracted quartet
+};
+
+// Boys table F_j(0.1 k), k = 0..400, j = 0..23 (positive-term series, long double).
+void make_boys_table(std::vector<double>& tab)
+{
+    tab.assign(kBoysRows * kBoysCols, 0.0);
+    for (int it = 0; it < kBoysRows; ++it) {
+        const long double T = 0.1L * it;
+        const long double eT = expl(-T);
+        for (int j = 0; j < kBoysCols; ++j) {
+            long double term = 1.0L / (2 * j + 1), sum = term;
+            for (int k = 1; k < 2000; ++k) {
+                term *= 2.0L * T / (2 * j + 2 * k + 1);
+                sum += term;
+                if (term < sum * 1e-21L) break;
+            }
+            tab[it * kBoysCols + j] = (double)(eT * sum);
+        }
+    }
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+__global__ void k_prescale(const double* __restrict__ D, const double* __restrict__ ns, int n, double* __restrict__ Dp,
+                           double* __restrict__ Jt, double* __restrict__ Kt, unsigned long long* __restrict__ dmax_bits)
+{
+    const long long nn = (long long)n * n;
+    double m = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(i / n), c = (int)(i % n);
+        const double v = D[i] * ns[r] * ns[c];
+        Dp[i] = v;
+        Jt[i] = 0.0;
+        Kt[i] = 0.0;
+        m = fmax(m, fabs(v));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(dmax_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_finalize(const double* __restrict__ Jt, const double* __restrict__ Kt, const double* __restrict__ ns,
+                           int n, double* __restrict__ J, double* __restrict__ K)
+{
+    const long long nn = (long long)n * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(i / n), c = (int)(i % n);
+        const long long t = (long long)c * n + r;
+        const double w = ns[r] * ns[c];
+        if (J) J[i] = (Jt[i] + Jt[t]) * w;
+        if (K) K[i] = (Kt[i] + Kt[t]) * w;
+    }
+}
+
+}  // namespace
+
+struct qcjk_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    qcjk_opts opts{};
+    int n = 0, nshell = 0;
+    std::vector<int> sl, snp, spoff, sbf;
+    std::vector<double> scen, exps, coefs, ns;
+    // pairs, sorted: class c occupies [coff[c], coff[c+1])
+    int coff[kNumPairClasses + 1] = {};
+    std::vector<int> pair_first, pair_second;  // shell indices per sorted pair
+    std::vector<double> qraw, qnorm;
+    std::vector<int> pair_np;
+    long long npairs = 0;
+    // device
+    double* d_boys = nullptr;
+    double *d_exps = nullptr, *d_coefs = nullptr, *d_cen = nullptr, *d_ns = nullptr;
+    int *d_spoff = nullptr, *d_snp = nullptr;
+    double4 *d_pp0 = nullptr, *d_pp1 = nullptr;
+    int4* d_pinfo = nullptr;
+    double4* d_pgeo = nullptr;
+    double *d_D = nullptr, *d_Dp = nullptr, *d_Jt = nullptr, *d_Kt = nullptr, *d_J = nullptr, *d_K = nullptr;
+    unsigned long long* d_dmax = nullptr;
+    unsigned long long* d_counters = nullptr;  // 3 per class pair
+    std::vector<ClassPairWork> work;
+    // per-class profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> pev;            // 2 per class pair
+    std::vector<qcjk_class_stat> ptot;      // running totals
+    bool pending = false;                   // events of the last build not yet folded into ptot
+    unsigned pflags = 0;
+};
+
+#define HIPCHK(x)                                          \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return QCJK_ERR_HIP;         \
+    } while (0)
+
+static void free_handle(qcjk_handle* h)
+{
+    if (!h) return;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto& w : h->work) { hipFree(w.d_start); hipFree(w.d_nsurv); }
+    void* ptrs[] = {h->d_boys, h->d_exps, h->d_coefs, h->d_cen, h->d_ns, h->d_spoff, h->d_snp, h->d_pp0, h->d_pp1,
+                    h->d_pinfo, h->d_pgeo, h->d_D, h->d_Dp, h->d_Jt, h->d_Kt, h->d_J, h->d_K, h->d_dmax, h->d_counters};
+    for (void* p : ptrs) if (p) hipFree(p);
+    for (hipEvent_t e : h->pev) hipEventDestroy(e);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static BuildArgs make_args(qcjk_handle* h, unsigned flags)
+{
+    BuildArgs a{};
+    a.D = h->d_Dp;
+    a.Jt = h->d_Jt;
+    a.Kt = h->d_Kt;
+    a.n = h->n;
+    a.flags = flags;
+    a.boys = h->d_boys;
+    a.pp0 = h->d_pp0;
+    a.pp1 = h->d_pp1;
+    a.pinfo = h->d_pinfo;
+    a.pgeo = h->d_pgeo;
+    a.dmax = reinterpret_cast<const double*>(h->d_dmax);
+    a.screen_tol = h->opts.screen_tol;
+    a.mixed_tol = h->opts.mixed_tol;
+    a.counters = nullptr;
+    return a;
+}
+
+static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
+{
+    h->opts = *o;
+    h->device = o->device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return QCJK_ERR_NODEVICE;
+    if (o->device < 0 || o->device >= ndev) return QCJK_ERR_ARG;
+    HIPCHK(hipSetDevice(o->device));
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&h->ev0));
+    HIPCHK(hipEventCreate(&h->ev1));
+    set_force_heavy(std::getenv("QCJK_FORCE_HEAVY") != nullptr);
+
+    const int ns = b->nshell;
+    h->nshell = ns;
+    h->sl.assign(b->shell_l, b->shell_l + ns);
+    h->snp.assign(b->shell_nprim, b->shell_nprim + ns);
+    h->spoff.resize(ns);
+    h->sbf.resize(ns);
+    int np = 0, nbf = 0;
+    for (int s = 0; s < ns; ++s) {
+        h->spoff[s] = np;
+        h->sbf[s] = nbf;
+        np += h->snp[s];
+        nbf += ncart(h->sl[s]);
+    }
+    h->n = nbf;
+    h->scen.assign(b->shell_center, b->shell_center + 3 * ns);
+    h->exps.assign(b->prim_exp, b->prim_exp + np);
+    h->coefs.assign(b->prim_coef, b->prim_coef + np);
+    h->ns.assign(b->bf_norm, b->bf_norm + nbf);
+    for (int i = 0; i < np; ++i)
+        if (!(h->exps[i] > 0.0) || !std::isfinite(h->coefs[i])) return QCJK_ERR_ARG;
+    for (int i = 0; i < nbf; ++i)
+        if (!(h->ns[i] > 0.0) || !std::isfinite(h->ns[i])) return QCJK_ERR_ARG;
+
+    // Boys table + basis arrays
+    std::vector<double> tab;
+    make_boys_table(tab);
+    HIPCHK(dalloc(&h->d_boys, tab.size()));
+    HIPCHK(hipMemcpy(h->d_boys, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(dalloc(&h->d_exps, np));
+    HIPCHK(dalloc(&h->d_coefs, np));
+    HIPCHK(dalloc(&h->d_cen, 3 * ns));
+    HIPCHK(dalloc(&h->d_ns, nbf));
+    HIPCHK(dalloc(&h->d_spoff, ns));
+    HIPCHK(dalloc(&h->d_snp, ns));
+    HIPCHK(hipMemcpy(h->d_exps, h->exps.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_coefs, h->coefs.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_cen, h->scen.data(), 3 * ns * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_ns, h->ns.data(), nbf * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_spoff, h->spoff.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_snp, h->snp.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+
+    // unique shell pairs, oriented (higher l first; equal l: higher index first), grouped by class
+    std::vector<std::vector<int>> cls_first(kNumPairClasses), cls_second(kNumPairClasses);
+    for (int A = 0; A < ns; ++A)
+        for (int B = 0; B <= A; ++B) {
+            int f = A, s = B;
+            if (h->sl[A] < h->sl[B]) std::swap(f, s);
+            const int c = pair_class(h->sl[f], h->sl[s]);
+            cls_first[c].push_back(f);
+            cls_second[c].push_back(s);
+        }
+    std::vector<int> pf, psd;
+    for (int c = 0; c < kNumPairClasses; ++c) {
+        h->coff[c] = (int)pf.size();
+        pf.insert(pf.end(), cls_first[c].begin(), cls_first[c].end());
+        psd.insert(psd.end(), cls_second[c].begin(), cls_second[c].end());
+    }
+    h->coff[kNumPairClasses] = (int)pf.size();
+    const int npair = (int)pf.size();
+    h->npairs = npair;
+
+    // primitive-pair precompute
+    std::vector<int4> pshell(npair);
+    std::vector<int2> pnp(npair);
+    std::vector<int> ppoff(npair);
+    long long npp = 0;
+    for (int k = 0; k < npair; ++k) {
+        pshell[k] = make_int4(pf[k], psd[k], h->spoff[pf[k]], h->spoff[psd[k]]);
+        pnp[k] = make_int2(h->snp[pf[k]], h->snp[psd[k]]);
+        ppoff[k] = (int)npp;
+        npp += (long long)h->snp[pf[k]] * h->snp[psd[k]];
+    }
+    if (npp > 0x7fffffffLL) return QCJK_ERR_UNSUPPORTED;
+    int4* d_pshell = nullptr;
+    int2* d_pnp = nullptr;
+    int* d_ppoff = nullptr;
+    HIPCHK(dalloc(&d_pshell, npair));
+    HIPCHK(dalloc(&d_pnp, npair));
+    HIPCHK(dalloc(&d_ppoff, npair));
+    HIPCHK(dalloc(&h->d_pp0, npp));
+    HIPCHK(dalloc(&h->d_pp1, npp));
+    HIPCHK(hipMemcpy(d_pshell, pshell.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_pnp, pnp.data(), npair * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ppoff, ppoff.data(), npair * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(launch_pair_setup(d_pshell, d_pnp, h->d_exps, h->d_coefs, h->d_cen, npair, h->d_pp0, h->d_pp1, d_ppoff,
+                             h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    hipFree(d_pshell);
+    hipFree(d_pnp);
+    hipFree(d_ppoff);
+
+    // unsorted pair records -> Schwarz
+    auto make_pinfo = [&](int k) {
+        const int f = pf[k], s = psd[k];
+        const int npr = h->snp[f] * h->snp[s];
+        return make_int4(h->sbf[f], h->sbf[s], ppoff[k], npr | ((f == s ? 1 : 0) << 16));
+    };
+    auto make_pgeo = [&](int k, double q) {
+        const int f = pf[k], s = psd[k];
+        return make_double4(h->scen[3 * f] - h->scen[3 * s], h->scen[3 * f + 1] - h->scen[3 * s + 1],
+                            h->scen[3 * f + 2] - h->scen[3 * s + 2], q);
+    };
+    std::vector<int4> pinfo(npair);
+    std::vector<double4> pgeo(npair);
+    for (int k = 0; k < npair; ++k) {
+        if (h->snp[pf[k]] * h->snp[psd[k]] > 0xffff) return QCJK_ERR_UNSUPPORTED;
+        pinfo[k] = make_pinfo(k);
+        pgeo[k] = make_pgeo(k, 0.0);
+    }
+    HIPCHK(dalloc(&h->d_pinfo, npair));
+    HIPCHK(dalloc(&h->d_pgeo, npair));
+    HIPCHK(hipMemcpy(h->d_pinfo, pinfo.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_pgeo, pgeo.data(), npair * sizeof(double4), hipMemcpyHostToDevice));
+    double *d_qraw = nullptr, *d_qnorm = nullptr;
+    HIPCHK(dalloc(&d_qraw, npair));
+    HIPCHK(dalloc(&d_qnorm, npair));
+    {
+        BuildArgs a = make_args(h, 0);
+        for (int c = 0; c < kNumPairClasses; ++c) {
+            const int off = h->coff[c], cnt = h->coff[c + 1] - h->coff[c];
+            HIPCHK(launch_schwarz(c, a, off, cnt, h->d_ns, d_qraw + off, d_qnorm + off, h->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> qraw(npair), qnorm(npair);
+    HIPCHK(hipMemcpy(qraw.data(), d_qraw, npair * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(qnorm.data(), d_qnorm, npair * sizeof(double), hipMemcpyDeviceToHost));
+    hipFree(d_qraw);
+    hipFree(d_qnorm);
+
+    // sort each class by descending raw bound; rebuild records in sorted order
+    std::vector<int> order(npair);
+    std::iota(order.begin(), order.end(), 0);
+    for (int c = 0; c < kNumPairClasses; ++c)
+        std::stable_sort(order.begin() + h->coff[c], order.begin() + h->coff[c + 1],
+                         [&](int x, int y) { return qraw[x] > qraw[y]; });
+    h->pair_first.resize(npair);
+    h->pair_second.resize(npair);
+    h->qraw.resize(npair);
+    h->qnorm.resize(npair);
+    h->pair_np.resize(npair);
+    for (int k = 0; k < npair; ++k) {
+        const int src = order[k];
+        pinfo[k] = make_pinfo(src);
+        pgeo[k] = make_pgeo(src, qraw[src]);
+        h->pair_first[k] = pf[src];
+        h->pair_second[k] = psd[src];
+        h->qraw[k] = qraw[src];
+        h->qnorm[k] = qnorm[src];
+        h->pair_np[k] = h->snp[pf[src]] * h->snp[psd[src]];
+    }
+    HIPCHK(hipMemcpy(h->d_pinfo, pinfo.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_pgeo, pgeo.data(), npair * sizeof(double4), hipMemcpyHostToDevice));
+
+    // work lists per (bra class >= ket class)
+    for (int bc = 0; bc < kNumPairClasses; ++bc)
+        for (int kc = 0; kc <= bc; ++kc) {
+            const int nb = h->coff[bc + 1] - h->coff[bc], nk = h->coff[kc + 1] - h->coff[kc];
+            if (nb == 0 || nk == 0) continue;
+            ClassPairWork w;
+            w.bra = bc;
+            w.ket = kc;
+            w.info = class_kernel(bc, kc);
+            if (!w.info) return QCJK_ERR_UNSUPPORTED;
+            w.tile = w.info->tile;
+            w.prim_flops = prim_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
+            w.digest_flops = digest_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
+            std::vector<long long> start(nb + 1, 0);
+            std::vector<int> nsurv(nb, 0);
+            const double* qk = h->qraw.data() + h->coff[kc];
+            for (int P = 0; P < nb; ++P) {
+                const double qp = h->qraw[h->coff[bc] + P];
+                // ket list is sorted descending: survivors are a prefix
+                int lo = 0, hi = nk;
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (qp * qk[mid] >= kPruneTol) lo = mid + 1;
+                    else hi = mid;
+                }
+                int m = lo;
+                if (bc == kc) m = std::min(m, P + 1);
+                nsurv[P] = m;
+                start[P + 1] = start[P] + (m + w.tile - 1) / w.tile;
+                w.nquartets += m;
+            }
+            w.nunits = start[nb];
+            HIPCHK(dalloc(&w.d_start, nb + 1));
+            HIPCHK(dalloc(&w.d_nsurv, nb));
+            HIPCHK(hipMemcpy(w.d_start, start.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.d_nsurv, nsurv.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+            h->work.push_back(w);
+        }
+
+    const size_t nn = (size_t)nbf * nbf;
+    HIPCHK(dalloc(&h->d_D, nn));
+    HIPCHK(dalloc(&h->d_Dp, nn));
+    HIPCHK(dalloc(&h->d_Jt, nn));
+    HIPCHK(dalloc(&h->d_Kt, nn));
+    HIPCHK(dalloc(&h->d_J, nn));
+    HIPCHK(dalloc(&h->d_K, nn));
+    HIPCHK(dalloc(&h->d_dmax, 1));
+    HIPCHK(dalloc(&h->d_counters, 3 * h->work.size() + 3));
+    return QCJK_OK;
+}
+
+// Fold the previous profiled build's events and counters into the running totals.
+static int fold_profile(qcjk_handle* h)
+{
+    if (!h->pending) return QCJK_OK;
+    h->pending = false;
+    const int nctr = (int)(3 * h->work.size() + 3);
+    std::vector<unsigned long long> ctr(nctr);
+    HIPCHK(hipEventSynchronize(h->pev.back()));
+    HIPCHK(hipMemcpy(ctr.data(), h->d_counters, nctr * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h->work.size(); ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->pev[2 * i], h->pev[2 * i + 1]));
+        qcjk_class_stat& t = h->ptot[i];
+        t.launches += 1;
+        t.quartets += (long long)ctr[3 * i];
+        t.prim_quartets += (long long)ctr[3 * i + 2];
+        t.flops += (double)ctr[3 * i + 2] * h->work[i].prim_flops + (double)ctr[3 * i] * h->work[i].digest_flops;
+        t.device_ms += ms;
+    }
+    return QCJK_OK;
+}
+
+extern "C" {
+
+int qcjk_profile(qcjk_handle* h, int enable)
+{
+    if (!h) return QCJK_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    if (h->pev.empty()) {
+        h->pev.resize(2 * h->work.size());
+        for (auto& e : h->pev) HIPCHK(hipEventCreate(&e));
+    }
+    h->pending = false;
+    h->profiling = enable != 0;
+    h->ptot.assign(h->work.size(), qcjk_class_stat{});
+    for (size_t i = 0; i < h->work.size(); ++i) {
+        const ClassKernelInfo* k = h->work[i].info;
+        h->ptot[i].la = k->la; h->ptot[i].lb = k->lb; h->ptot[i].lc = k->lc; h->ptot[i].ld = k->ld;
+        h->ptot[i].wavefront_per_quartet = k->tile == 1 ? 1 : 0;
+    }
+    return QCJK_OK;
+}
+
+int qcjk_class_report(qcjk_handle* h, qcjk_class_stat* out, int max, int* count)
+{
+    if (!h || !count) return QCJK_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    const int rc = fold_profile(h);
+    if (rc != QCJK_OK) return rc;
+    *count = (int)h->ptot.size();
+    for (int i = 0; i < (int)h->ptot.size() && i < max && out; ++i) out[i] = h->ptot[i];
+    return QCJK_OK;
+}
+
+int qcjk_create(const qcjk_basis* basis, const qcjk_opts* opts, qcjk_handle** out)
+{
+    if (!out) return QCJK_ERR_ARG;
+    *out = nullptr;
+    if (!basis || basis->nshell <= 0 || !basis->shell_l || !basis->shell_nprim || !basis->shell_center ||
+        !basis->prim_exp || !basis->prim_coef || !basis->bf_norm)
+        return QCJK_ERR_ARG;
+    if (basis->order != 0) return QCJK_ERR_UNSUPPORTED;
+    for (int s = 0; s < basis->nshell; ++s) {
+        if (basis->shell_l[s] < 0 || basis->shell_l[s] > QCJK_MAX_L) return QCJK_ERR_UNSUPPORTED;
+        if (basis->shell_nprim[s] < 1 || basis->shell_nprim[s] > QCJK_MAX_NPRIM) return QCJK_ERR_UNSUPPORTED;
+    }
+    qcjk_opts o{};
+    if (opts) o = *opts;
+    qcjk_handle* h = new (std::nothrow) qcjk_handle();
+    if (!h) return QCJK_ERR_NOMEM;
+    const int rc = create_impl(basis, &o, h);
+    if (rc != QCJK_OK) {
+        free_handle(h);
+        return rc;
+    }
+    *out = h;
+    return QCJK_OK;
+}
+
+int qcjk_nbf(const qcjk_handle* h) { return h ? h->n : -1; }
+
+int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags, void* stream,
+                      qcjk_stats* stats)
+{
+    if (!h || !dD) return QCJK_ERR_ARG;
+    if (!dJ) flags &= ~(unsigned)QCJK_J;
+    if (!dK) flags &= ~(unsigned)QCJK_K;
+    if (flags & QCJK_MIXED) return QCJK_ERR_UNSUPPORTED;
+    HIPCHK(hipSetDevice(h->device));
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const int n = h->n;
+    const long long nn = (long long)n * n;
+    const int eb = (int)std::min<long long>((nn + 255) / 256, 4096);
+    const int nctr = (int)(3 * h->work.size() + 3);
+    if (h->profiling) {
+        const int rc = fold_profile(h);
+        if (rc != QCJK_OK) return rc;
+    }
+    if (stats) HIPCHK(hipEventRecord(h->ev0, s));
+    hipLaunchKernelGGL(k_zero_u64, dim3((nctr + 255) / 256), dim3(256), 0, s, h->d_counters, nctr);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, h->d_dmax, 1);
+    hipLaunchKernelGGL(k_prescale, dim3(eb), dim3(256), 0, s, dD, h->d_ns, n, h->d_Dp, h->d_Jt, h->d_Kt, h->d_dmax);
+    HIPCHK(hipGetLastError());
+    if (flags & (QCJK_J | QCJK_K)) {
+        BuildArgs a = make_args(h, flags);
+        for (size_t i = 0; i < h->work.size(); ++i) {
+            const ClassPairWork& w = h->work[i];
+            ClassPairArgs c{};
+            c.bra_off = h->coff[w.bra];
+            c.nbra = h->coff[w.bra + 1] - h->coff[w.bra];
+            c.ket_off = h->coff[w.ket];
+            c.nket = h->coff[w.ket + 1] - h->coff[w.ket];
+            c.same = w.bra == w.ket;
+            c.unit_start = w.d_start;
+            c.nsurv = w.d_nsurv;
+            c.nunits = w.nunits;
+            c.tile = w.tile;
+            a.counters = h->d_counters + 3 * i;
+            if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i], s));
+            HIPCHK(w.info->launch(a, c, s));
+            if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i + 1], s));
+        }
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(eb), dim3(256), 0, s, h->d_Jt, h->d_Kt, h->d_ns, n,
+                       (flags & QCJK_J) ? dJ : nullptr, (flags & QCJK_K) ? dK : nullptr);
+    HIPCHK(hipGetLastError());
+    if (h->profiling && (flags & (QCJK_J | QCJK_K))) h->pending = true;
+    if (stats) {
+        HIPCHK(hipEventRecord(h->ev1, s));
+        HIPCHK(hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        std::vector<unsigned long long> ctr(nctr);
+        HIPCHK(hipMemcpy(ctr.data(), h->d_counters, nctr * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->quartets_total = h->npairs * (h->npairs + 1) / 2;
+        for (size_t i = 0; i < h->work.size(); ++i) {
+            stats->quartets_computed += (long long)ctr[3 * i];
+            stats->quartets_fp32 += (long long)ctr[3 * i + 1];
+            stats->prim_quartets += (long long)ctr[3 * i + 2];
+            stats->flops += (double)ctr[3 * i + 2] * h->work[i].prim_flops + (double)ctr[3 * i] * h->work[i].digest_flops;
+        }
+        stats->device_ms = ms;
+    }
+    return QCJK_OK;
+}
+
+int qcjk_build(qcjk_handle* h, const double* D, double* J, double* K, unsigned flags, qcjk_stats* stats)
+{
+    if (!h || !D) return QCJK_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    const size_t bytes = (size_t)h->n * h->n * sizeof(double);
+    HIPCHK(hipMemcpyAsync(h->d_D, D, bytes, hipMemcpyHostToDevice, h->stream));
+    const bool wantJ = J && (flags & QCJK_J), wantK = K && (flags & QCJK_K);
+    const int rc = qcjk_build_device(h, h->d_D, wantJ ? h->d_J : nullptr, wantK ? h->d_K : nullptr, flags, h->stream,
+                                     stats);
+    if (rc != QCJK_OK) return rc;
+    // outputs not requested are left untouched
+    if (wantJ) HIPCHK(hipMemcpyAsync(J, h->d_J, bytes, hipMemcpyDeviceToHost, h->stream));
+    if (wantK) HIPCHK(hipMemcpyAsync(K, h->d_K, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return QCJK_OK;
+}
+
+int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* R, double* S, double* T, double* V)
+{
+    if (!h || natom < 0 || (natom > 0 && (!Z || !R))) return QCJK_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    const int n = h->n;
+    const size_t nn = (size_t)n * n;
+    double *dZ = nullptr, *dR = nullptr, *dS = nullptr, *dT = nullptr, *dV = nullptr;
+    int4* dsp = nullptr;
+    int rc = QCJK_OK;
+    std::vector<int4> sp;
+    struct OE { int la, lb, start, count; };
+    std::vector<OE> groups;
+    for (int la = 0; la <= QCJK_MAX_L; ++la)
+        for (int lb = 0; lb <= la; ++lb) {
+            const int start = (int)sp.size();
+            for (int A = 0; A < h->nshell; ++A)
+                for (int B = 0; B < h->nshell; ++B) {
+                    if (h->sl[A] != la || h->sl[B] != lb) continue;
+                    if (la == lb && B > A) continue;
+                    sp.push_back(make_int4(A, B, h->sbf[A], h->sbf[B]));
+                }
+            groups.push_back({la, lb, start, (int)sp.size() - start});
+        }
+    auto fail = [&](hipError_t e) { if (e != hipSuccess && rc == QCJK_OK) rc = QCJK_ERR_HIP; };
+    fail(dalloc(&dZ, natom));
+    fail(dalloc(&dR, 3 * natom));
+    fail(dalloc(&dS, nn));
+    fail(dalloc(&dT, nn));
+    fail(dalloc(&dV, nn));
+    fail(dalloc(&dsp, sp.size()));
+    if (rc == QCJK_OK) {
+        if (natom > 0) {
+            fail(hipMemcpy(dZ, Z, natom * sizeof(double), hipMemcpyHostToDevice));
+            fail(hipMemcpy(dR, R, 3 * natom * sizeof(double), hipMemcpyHostToDevice));
+        }
+        fail(hipMemcpy(dsp, sp.data(), sp.size() * sizeof(int4), hipMemcpyHostToDevice));
+        for (const OE& g : groups)
+            if (rc == QCJK_OK)
+                fail(launch_one_electron(g.la, g.lb, dsp + g.start, g.count, h->d_spoff, h->d_snp, h->d_exps,
+                                         h->d_coefs, h->d_cen, h->d_ns, n, natom, dZ, dR, h->d_boys, dS, dT, dV,
+                                         h->stream));
+        fail(hipStreamSynchronize(h->stream));
+        if (S) fail(hipMemcpy(S, dS, nn * sizeof(double), hipMemcpyDeviceToHost));
+        if (T) fail(hipMemcpy(T, dT, nn * sizeof(double), hipMemcpyDeviceToHost));
+        if (V) fail(hipMemcpy(V, dV, nn * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    hipFree(dZ); hipFree(dR); hipFree(dS); hipFree(dT); hipFree(dV); hipFree(dsp);
+    return rc;
+}
+
+int qcjk_schwarz(qcjk_handle* h, double* Q)
+{
+    if (!h || !Q) return QCJK_ERR_ARG;
+    const int ns = h->nshell;
+    for (int i = 0; i < ns * ns; ++i) Q[i] = 0.0;
+    for (long long k = 0; k < h->npairs; ++k) {
+        const int a = h->pair_first[k], b = h->pair_second[k];
+        Q[a * ns + b] = Q[b * ns + a] = h->qnorm[k];
+    }
+    return QCJK_OK;
+}
+
+void qcjk_destroy(qcjk_handle* h) { free_handle(h); }
+
+const char* qcjk_strerror(int code)
+{
+    switch (code) {
+    case QCJK_OK: return "ok";
+    case QCJK_ERR_ARG: return "invalid argument";
+    case QCJK_ERR_UNSUPPORTED: return "unsupported basis or option";
+    case QCJK_ERR_HIP: return "HIP runtime error";
+    case QCJK_ERR_NOMEM: return "out of memory";
+    case QCJK_ERR_NODEVICE: return "no usable HIP device";
+    default: return "unknown error";
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// qcjk_internal.h -- device-side data layout shared by the kernels and the C-ABI layer.
+//
+// HBM layout (all resident for the life of a handle; nothing O(N^4) is stored):
+//   shell pairs    : one global array, grouped by pair class (la>=lb: ss, ps, pp, ds, dp, dd), each
+//                    class sorted by descending Schwarz bound.  Per pair an int4 {bf offset of the
+//                    first shell, bf offset of the second, first primitive pair, nprim pairs | same
+//                    shell flag << 16} and a double4 {A-B (x,y,z), Q_raw}.
+//   primitive pairs: two double4 per primitive pair: {P (x,y,z), p} and {P-A (x,y,z), c_i c_j K_AB}.
+//   density / Fock : D' = ns (x) ns o D (n x n), the unsymmetrised accumulators Jt, Kt (n x n).
+//   Boys table     : 401 x 24 FP64 (77 KB).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qcjk {
+
+constexpr int kNumPairClasses = 6;  // (0,0) (1,0) (1,1) (2,0) (2,1) (2,2)
+constexpr int kPairClassL[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+constexpr int pair_class(int la, int lb) { return la == 0 ? 0 : (la == 1 ? 1 + lb : 3 + lb); }
+
+enum : unsigned { kFlagJ = 1u, kFlagK = 2u, kFlagMixed = 4u, kFlagScreen = 8u };
+
+struct BuildArgs {
+    const double* D;       // D' (n x n), ns-prescaled
+    double* Jt;            // J accumulator (n x n), symmetrised later
+    double* Kt;            // K accumulator
+    int n;
+    unsigned flags;
+    const double* boys;    // kBoysRows x kBoysCols
+    const double4* pp0;    // {Px,Py,Pz,p}
+    const double4* pp1;    // {PAx,PAy,PAz,pref}
+    const int4* pinfo;     // per shell pair
+    const double4* pgeo;   // per shell pair {ABx,ABy,ABz,Qraw}
+    const double* dmax;    // device scalar max|D'|
+    double screen_tol;
+    double mixed_tol;
+    unsigned long long* counters;  // [0] quartets computed, [1] fp32 quartets, [2] prim quartets
+};
+
+struct ClassPairArgs {
+    int bra_off, nbra;     // bra class range in the pair array
+    int ket_off, nket;     // ket class range
+    int same;              // bra class == ket class -> ket index <= bra index
+    const long long* unit_start;  // [nbra+1] prefix sums of work units per bra pair
+    const int* nsurv;             // [nbra] surviving ket pairs (a prefix of the ket class list)
+    long long nunits;
+    int tile;              // ket pairs per unit (64 for the lane-per-quartet kernels, 1 otherwise)
+};
+
+// Launch one class-pair kernel.  Returns hipError_t.
+typedef hipError_t (*ClassLauncher)(const BuildArgs&, const ClassPairArgs&, hipStream_t);
+
+struct ClassKernelInfo {
+    int la, lb, lc, ld;
+    int tile;              // 64 = lane per quartet, 1 = wavefront per quartet
+    ClassLauncher launch;
+};
+
+// Table of the 21 bra-class >= ket-class combinations (qcjk_eri.hip).
+const ClassKernelInfo* class_kernel(int bra_class, int ket_class);
+void set_force_heavy(bool on);  // route every class through the wavefront-per-quartet kernel
+
+// Setup kernels (qcjk_setup.hip).
+hipError_t launch_pair_setup(const int4* pair_shells /* {A,B,poffA,poffB} */, const int2* pair_np,
+                             const double* exps, const double* coefs, const double* centers,
+                             int npairs, double4* pp0, double4* pp1, const int* pp_off, hipStream_t s);
+hipError_t launch_schwarz(int pair_class, const BuildArgs& a, int off, int npairs, const double* ns,
+                          double* qraw, double* qnorm, hipStream_t s);
+hipError_t launch_one_electron(int la, int lb, const int4* spairs /* {A,B,bfA,bfB} */, int nsp,
+                               const int* shell_poff, const int* shell_np, const double* exps,
+                               const double* coefs, const double* centers, const double* ns, int n,
+                               int natom, const double* Z, const double* Rn, const double* boys, double* S,
+                               double* T, double* V, hipStream_t s);
+
+}  // namespace qcjk

@@ -1,0 +1,224 @@
+// scf.cpp -- see scf.hpp.
+#include "scf.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <deque>
+#include <stdexcept>
+
+namespace qchem_amd {
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// Orthogonaliser: S = U s U^T, V = U s^{-1/2} over the kept eigenvalues, Vinv = V^T S (V^T S V = 1).
+struct Ortho {
+    Mat V, VtS;
+    int m = 0;  // kept dimension
+    void set(const Mat& S)
+    {
+        std::vector<double> s;
+        Mat U;
+        sym_eigen(S, s, U);
+        const int n = S.n;
+        const double smax = s.empty() ? 0.0 : s.back();
+        std::vector<int> keep;
+        for (int k = 0; k < n; ++k)
+            if (s[k] > 1e-10 * smax) keep.push_back(k);
+        m = (int)keep.size();
+        V = Mat(n);  // n x n storage, columns beyond m stay zero
+        for (int c = 0; c < m; ++c) {
+            const double f = 1.0 / std::sqrt(s[keep[c]]);
+            for (int i = 0; i < n; ++i) V(i, c) = U(i, keep[c]) * f;
+        }
+        VtS = transpose(V) * S;
+    }
+    // F' = V^T F V restricted to m x m
+    Mat to_ortho(const Mat& F) const
+    {
+        Mat full = transpose(V) * F * V;
+        Mat r(m);
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) r(i, j) = full(i, j);
+        return r;
+    }
+    // D' = (V^T S) D (V^T S)^T restricted to m x m
+    Mat density_to_ortho(const Mat& D) const
+    {
+        Mat full = VtS * D * transpose(VtS);
+        Mat r(m);
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) r(i, j) = full(i, j);
+        return r;
+    }
+};
+
+// Pulay DIIS on the orthonormal Fock (SCFAcceleratorDIIS.C:35-212, Math/DIIS.C:24-51).
+struct DIIS {
+    int nproj;
+    double fd_max, sv_tol;
+    std::deque<Mat> F, E;
+
+    Mat next(const Mat& Fp, const Mat& Ep, double en)
+    {
+        if (en > fd_max) return Fp;  // "FD>FDMax": plain diagonalisation
+        F.push_back(Fp);
+        E.push_back(Ep);
+        if ((int)F.size() > nproj) { F.pop_front(); E.pop_front(); }
+        if (F.size() < 2) return Fp;  // "Nproj<2"
+        std::vector<double> c;
+        for (;;) {
+            const int N = (int)F.size();
+            const int nb = N + 1;
+            Mat Bb(nb);
+            for (int i = 0; i < N; ++i) {
+                for (int j = 0; j < N; ++j) Bb(i, j) = dot(E[i], E[j]);
+                Bb(i, N) = Bb(N, i) = 1.0;
+            }
+            // smallest singular value of the symmetric bordered matrix = min |eigenvalue|
+            std::vector<double> w;
+            Mat U;
+            sym_eigen(Bb, w, U);
+            double sv = 1e300;
+            for (double x : w) sv = std::min(sv, std::fabs(x));
+            if (sv < sv_tol && N >= 2) {
+                F.pop_front();
+                E.pop_front();
+                continue;
+            }
+            if (nb <= 2) return Fp;  // "B.rows()<=2"
+            std::vector<double> rhs(nb, 0.0), x;
+            rhs[N] = 1.0;
+            if (!solve(Bb.a, rhs, nb, x)) return Fp;
+            c.assign(x.begin(), x.begin() + N);
+            break;
+        }
+        Mat r(Fp.n);
+        for (size_t i = 0; i < F.size(); ++i)
+            for (size_t k = 0; k < r.a.size(); ++k) r.a[k] += c[i] * F[i].a[k];
+        return r;
+    }
+};
+
+Mat density_from(const Ortho& X, const Mat& Fp, int nocc, std::vector<double>& eps, double& t_diag)
+{
+    const auto t0 = clk::now();
+    Mat Cp;
+    sym_eigen(Fp, eps, Cp);
+    t_diag += secs(t0, clk::now());
+    const int n = X.V.n, m = X.m;
+    // C = V C' (n x m), D = 2 sum_{occ} C C^T
+    Mat D(n);
+    std::vector<double> col(n);
+    for (int k = 0; k < nocc; ++k) {
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < m; ++j) s += X.V(i, j) * Cp(j, k);
+            col[i] = s;
+        }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) D(i, j) += 2.0 * col[i] * col[j];
+    }
+    return D;
+}
+
+}  // namespace
+
+double nuclear_repulsion(const std::vector<Atom>& atoms)
+{
+    double e = 0.0;
+    for (size_t i = 0; i < atoms.size(); ++i)
+        for (size_t j = 0; j < i; ++j) {
+            const double dx = atoms[i].R[0] - atoms[j].R[0], dy = atoms[i].R[1] - atoms[j].R[1],
+                         dz = atoms[i].R[2] - atoms[j].R[2];
+            e += atoms[i].Z * atoms[j].Z / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    return e;
+}
+
+void one_electron(GpuHF_IBS& eng, const std::vector<Atom>& atoms, Mat& S, Mat& h)
+{
+    const int n = eng.nbf();
+    std::vector<double> Z(atoms.size()), R(3 * atoms.size());
+    for (size_t i = 0; i < atoms.size(); ++i) {
+        Z[i] = atoms[i].Z;
+        for (int x = 0; x < 3; ++x) R[3 * i + x] = atoms[i].R[x];
+    }
+    S = Mat(n);
+    Mat T(n), V(n);
+    const int rc = qcjk_one_electron(eng.handle(), (int)atoms.size(), Z.data(), R.data(), S.a.data(), T.a.data(), V.a.data());
+    if (rc != QCJK_OK) throw EngineError("qcjk_one_electron", rc);
+    h = axpby(1.0, T, 1.0, V);
+}
+
+SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0)
+{
+    SCFResult res;
+    const auto t_start = clk::now();
+    if (nelec <= 0 || nelec % 2) throw std::runtime_error("rhf: closed-shell electron count required");
+    const int nocc = nelec / 2;
+    const int n = eng.nbf();
+    Mat S, h;
+    one_electron(eng, atoms, S, h);
+    Ortho X;
+    X.set(S);
+    if (nocc > X.m) throw std::runtime_error("rhf: more occupied orbitals than basis functions");
+    res.e_nuc = nuclear_repulsion(atoms);
+    double Ztot = 0.0;
+    for (const Atom& a : atoms) Ztot += a.Z;
+    DIIS diis{par.nproj, par.fd_max > 0 ? par.fd_max : (double)((int)Ztot * (int)Ztot) * 0.1 / 32, par.sv_tol, {}, {}};
+
+    // The Fock operator terms as the reference's Hamiltonian holds them: the two HF terms reach the
+    // basis through the WholeSystemFock face (J first, then K, same density -> one fused build).
+    const WholeSystemFock_IBS<double>& whole = eng;
+    Mat D = D0 ? *D0 : density_from(X, X.to_ortho(h), nocc, res.eps, res.t_diag);
+    if (D.n != n) throw std::runtime_error("rhf: seed density has the wrong dimension");
+    res.t_setup = secs(t_start, clk::now());
+
+    double E_old = 0.0;
+    for (int it = 1; it <= par.max_iter; ++it) {
+        const auto tj = clk::now();
+        Mat Dao(n);
+        whole.AddAODensity(Dao, D);
+        const Mat J = whole.MakeAOFock(Dao, false);
+        const Mat K = whole.MakeAOFock(Dao, true);
+        res.t_jk += secs(tj, clk::now());
+        Mat F = h;
+        whole.SliceAOFock(F, J);
+        Mat mhK = K;
+        mhK *= -0.5;
+        whole.SliceAOFock(F, mhK);
+        res.e_one = dot(D, h);
+        res.e_coul = 0.5 * dot(D, J);
+        res.e_exch = -0.25 * dot(D, K);
+        const double E = res.e_one + res.e_coul + res.e_exch + res.e_nuc;
+        res.energy = E;
+        res.energy_history.push_back(E);
+        res.time_history.push_back(secs(t_start, clk::now()));
+        res.iterations = it;
+
+        const Mat Fp = X.to_ortho(F);
+        const Mat Dp = X.density_to_ortho(D);
+        const Mat Ep = axpby(1.0, Fp * Dp, -1.0, Dp * Fp);
+        const double fd = frob(Ep);
+        const double dE = it > 1 ? std::fabs(E - E_old) : 1e300;
+        if (par.verbose)
+            std::fprintf(stderr, "  it %3d  E = %.12f  dE = %.3e  [F,D] = %.3e\n", it, E, it > 1 ? E - E_old : 0.0, fd);
+        if (dE < par.e_tol && fd < par.fd_tol) {
+            res.converged = true;
+            res.D = D;
+            break;
+        }
+        E_old = E;
+        const Mat Fx = diis.next(Fp, Ep, fd);
+        D = density_from(X, Fx, nocc, res.eps, res.t_diag);
+        res.D = D;
+    }
+    res.t_total = secs(t_start, clk::now());
+    return res;
+}
+
+}  // namespace qchem_amd
