@@ -56,6 +56,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the per-class kernel-variant autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU path is design-only (--gpus %d requested)"
@@ -82,6 +83,11 @@ def main():
     stream = torch.cuda.current_stream(dev)
 
     flags = qa.QCJK_J | qa.QCJK_K
+    tuning = None
+    if not args.no_tune:  # untimed setup: pick the faster kernel variant per class on this workload
+        from qchem_amd import tune
+        tuning = tune.tune(eng, D, reps=2, cache=os.path.join(ROOT, "gpurun_out", "tuning_gfx950.json"),
+                           workload="random-cluster-16")
     st = eng.build_device(D, J, K, flags, stream=stream, stats=True)  # counts + first build
     for _ in range(args.warmup):
         eng.build_device(D, J, K, flags, stream=stream)
@@ -122,6 +128,9 @@ def main():
                       "n_basis": n, "quartets_per_build": quartets, "fock_build_s": ms / 1e3,
                       "parallelism": "single GPU"},
            "roofline": roof, "cpu_baseline": cpu,
+           "tuning": None if tuning is None else {"%d%d%d%d" % (r["la"], r["lb"], r["lc"], r["ld"]):
+                                                  [r["wavefront_per_quartet"], round(r["device_ms"], 3),
+                                                   round(r["alt_ms"], 3)] for r in tuning},
            "classes": [{k: r[k] for k in ("la", "lb", "lc", "ld", "wavefront_per_quartet", "quartets")} |
                        {"ms": r["device_ms"] / max(1, r["launches"]),
                         "tflops": r["flops"] / max(1e-12, r["device_ms"] / 1e3) / 1e12} for r in rep]}

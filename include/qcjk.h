@@ -124,10 +124,21 @@ typedef struct qcjk_class_stat {
     int64_t prim_quartets;
     double  flops;
     double  device_ms;
+    double  alt_ms;      /* qcjk_autotune: the other variant's time (-1 if none) */
 } qcjk_class_stat;
 
 int qcjk_profile(qcjk_handle* h, int enable);  /* enable != 0 also resets the totals */
 int qcjk_class_report(qcjk_handle* h, qcjk_class_stat* out, int max, int* count);
+
+/* Kernel variant per angular-momentum class (no reference counterpart): 0 = lane per contracted
+ * quartet (register resident; compiled for classes with L <= 4), 1 = wavefront per quartet (LDS
+ * staged; every class).  qcjk_autotune times both variants of every class present in this basis on
+ * the given DEVICE density (reps timed launches each, after one warm-up), keeps the faster one and
+ * reports, per class, the chosen variant in wavefront_per_quartet, its time in device_ms and the
+ * other variant's time in alt_ms (-1 when it has only one).  Results are cached by the caller
+ * (qchem_amd.tune) and re-applied with qcjk_set_variant. */
+int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant);
+int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* out, int max, int* count);
 
 void qcjk_destroy(qcjk_handle* h);
 

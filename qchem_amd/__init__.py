@@ -57,7 +57,8 @@ class Stats(ctypes.Structure):
 class ClassStat(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32), ("ld", ctypes.c_int32),
                 ("wavefront_per_quartet", ctypes.c_int32), ("launches", ctypes.c_int64), ("quartets", ctypes.c_int64),
-                ("prim_quartets", ctypes.c_int64), ("flops", ctypes.c_double), ("device_ms", ctypes.c_double)]
+                ("prim_quartets", ctypes.c_int64), ("flops", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("alt_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,7 +86,8 @@ class _ScfResult(ctypes.Structure):
 
 
 QCJK_SYMBOLS = ["qcjk_create", "qcjk_nbf", "qcjk_build", "qcjk_build_device", "qcjk_one_electron",
-                "qcjk_schwarz", "qcjk_profile", "qcjk_class_report", "qcjk_destroy", "qcjk_strerror"]
+                "qcjk_schwarz", "qcjk_profile", "qcjk_class_report", "qcjk_set_variant", "qcjk_autotune",
+                "qcjk_destroy", "qcjk_strerror"]
 HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qchost_last_error"]
 
 
@@ -121,6 +123,9 @@ def lib():
         L.qcjk_schwarz.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.qcjk_profile.argtypes = [vp, ctypes.c_int]
         L.qcjk_class_report.argtypes = [vp, ctypes.POINTER(ClassStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.qcjk_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.qcjk_autotune.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ClassStat), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)]
         L.qcjk_destroy.argtypes = [vp]
         L.qcjk_strerror.argtypes = [ctypes.c_int]
         L.qcjk_strerror.restype = ctypes.c_char_p
@@ -252,6 +257,27 @@ class JKEngine:
         if rc != 0:
             raise EngineError("qcjk_class_report", rc)
         return [arr[i].as_dict() for i in range(min(cnt.value, 64))]
+
+    def set_variant(self, la, lb, lc, ld, variant):
+        rc = lib().qcjk_set_variant(self.h, la, lb, lc, ld, variant)
+        if rc != 0:
+            raise EngineError("qcjk_set_variant", rc)
+
+    def autotune(self, dD, reps=3):
+        """Time both kernel variants per class on device density dD; keep the faster (see tune.py)."""
+        cnt = ctypes.c_int(0)
+        arr = (ClassStat * 64)()
+        ptr = dD if isinstance(dD, int) else dD.data_ptr()
+        rc = lib().qcjk_autotune(self.h, ptr, reps, arr, 64, ctypes.byref(cnt))
+        if rc != 0:
+            raise EngineError("qcjk_autotune", rc)
+        return [arr[i].as_dict() for i in range(min(cnt.value, 64))]
+
+    def apply_tuning(self, table):
+        """table: {"la lb lc ld": variant} (as written by qchem_amd.tune)."""
+        for key, v in table.items():
+            la, lb, lc, ld = (int(x) for x in key.split())
+            self.set_variant(la, lb, lc, ld, int(v))
 
     def schwarz(self):
         ns = len(self.basis.shells)

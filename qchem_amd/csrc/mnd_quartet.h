@@ -10,18 +10,29 @@ namespace qcjk {
 
 // Lane-per-quartet: all primitive quartets of (pi,pg | qi,qg) accumulated into I (registers),
 // raw (un-normalised) Cartesian integrals, layout I[((a*NB+b)*NC+c)*ND+d].
-template <int LA, int LB, int LC, int LD>
-QC_HD void light_compute(const BuildArgs& A, int4 pi, double4 pg, int4 qi, double4 qg, double* I)
+// With eab_mem (a wavefront-private LDS slot) the bra Hermite tables -- identical in every lane,
+// since a wavefront shares its bra pair -- are kept there instead of in registers: every lane
+// stores the same values and reads back its own stores, so no cross-lane ordering is needed.
+template <int LA, int LB, int LC, int LD, bool BRA_MEM = false>
+QC_HD void light_compute(const BuildArgs& A, int4 pi, double4 pg, int4 qi, double4 qg, double* I,
+                         HermiteAxis<LA, LB, double>* eab_mem = nullptr)
 {
     constexpr int L = LA + LB + LC + LD;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;
     for (int ij = 0; ij < npab; ++ij) {
         const double4 b0 = A.pp0[pi.z + ij], b1 = A.pp1[pi.z + ij];
-        HermiteAxis<LA, LB, double> eab[3];
+        HermiteAxis<LA, LB, double> eloc[3];
         const double a12 = 0.5 / b0.w;
-        eab[0].build(a12, b1.x, b1.x + pg.x);
-        eab[1].build(a12, b1.y, b1.y + pg.y);
-        eab[2].build(a12, b1.z, b1.z + pg.z);
+        eloc[0].build(a12, b1.x, b1.x + pg.x);
+        eloc[1].build(a12, b1.y, b1.y + pg.y);
+        eloc[2].build(a12, b1.z, b1.z + pg.z);
+        const HermiteAxis<LA, LB, double>* eab = eloc;
+        if constexpr (BRA_MEM) {
+            eloc[0].store(eab_mem[0]);
+            eloc[1].store(eab_mem[1]);
+            eloc[2].store(eab_mem[2]);
+            eab = eab_mem;
+        }
         for (int kl = 0; kl < npcd; ++kl) {
             const double4 k0 = A.pp0[qi.z + kl], k1 = A.pp1[qi.z + kl];
             HermiteAxis<LC, LD, double> ecd[3];

@@ -26,16 +26,21 @@ namespace {
 
 constexpr double kPruneTol = 1e-30;  // create-time: drop quartets whose Schwarz product is below this
 
+// Variants: 0 = lane per quartet (64 ket pairs per unit), 1 = wavefront per quartet (1 per unit).
+constexpr int kTile[2] = {64, 1};
+
 struct ClassPairWork {
     int bra, ket;
-    int tile;
+    int variant = 1;
     const ClassKernelInfo* info;
-    long long nunits = 0;
+    long long nunits[2] = {0, 0};
     long long nquartets = 0;   // surviving at create time
-    long long* d_start = nullptr;
+    long long* d_start[2] = {nullptr, nullptr};
     int* d_nsurv = nullptr;
     double prim_flops = 0;     // per primitive quartet
     double digest_flops = 0;   // per contracted quartet
+    bool has(int v) const { return v == 1 || info->light != nullptr; }
+    ClassLauncher launcher(int v) const { return v == 0 ? info->light : info->heavy; }
 };
 
 // Boys table F_j(0.1 k), k = 0..400, j = 0..23 (positive-term series, long double).
@@ -147,7 +152,7 @@ static void free_handle(qcjk_handle* h)
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (auto& w : h->work) { hipFree(w.d_start); hipFree(w.d_nsurv); }
+    for (auto& w : h->work) { hipFree(w.d_start[0]); hipFree(w.d_start[1]); hipFree(w.d_nsurv); }
     void* ptrs[] = {h->d_boys, h->d_exps, h->d_coefs, h->d_cen, h->d_ns, h->d_spoff, h->d_snp, h->d_pp0, h->d_pp1,
                     h->d_pinfo, h->d_pgeo, h->d_D, h->d_Dp, h->d_Jt, h->d_Kt, h->d_J, h->d_K, h->d_dmax, h->d_counters};
     for (void* p : ptrs) if (p) hipFree(p);
@@ -189,7 +194,7 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));
-    set_force_heavy(std::getenv("QCJK_FORCE_HEAVY") != nullptr);
+    const bool force_heavy = std::getenv("QCJK_FORCE_HEAVY") != nullptr;
 
     const int ns = b->nshell;
     h->nshell = ns;
@@ -355,10 +360,10 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
             w.ket = kc;
             w.info = class_kernel(bc, kc);
             if (!w.info) return QCJK_ERR_UNSUPPORTED;
-            w.tile = w.info->tile;
+            w.variant = (!force_heavy && w.info->tile == 64 && w.info->light) ? 0 : 1;
             w.prim_flops = prim_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             w.digest_flops = digest_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
-            std::vector<long long> start(nb + 1, 0);
+            std::vector<long long> start0(nb + 1, 0), start1(nb + 1, 0);
             std::vector<int> nsurv(nb, 0);
             const double* qk = h->qraw.data() + h->coff[kc];
             for (int P = 0; P < nb; ++P) {
@@ -373,13 +378,17 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
                 int m = lo;
                 if (bc == kc) m = std::min(m, P + 1);
                 nsurv[P] = m;
-                start[P + 1] = start[P] + (m + w.tile - 1) / w.tile;
+                start0[P + 1] = start0[P] + (m + kTile[0] - 1) / kTile[0];
+                start1[P + 1] = start1[P] + m;
                 w.nquartets += m;
             }
-            w.nunits = start[nb];
-            HIPCHK(dalloc(&w.d_start, nb + 1));
+            w.nunits[0] = start0[nb];
+            w.nunits[1] = start1[nb];
+            HIPCHK(dalloc(&w.d_start[0], nb + 1));
+            HIPCHK(dalloc(&w.d_start[1], nb + 1));
             HIPCHK(dalloc(&w.d_nsurv, nb));
-            HIPCHK(hipMemcpy(w.d_start, start.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.d_start[0], start0.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.d_start[1], start1.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(w.d_nsurv, nsurv.data(), nb * sizeof(int), hipMemcpyHostToDevice));
             h->work.push_back(w);
         }
@@ -393,6 +402,35 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     HIPCHK(dalloc(&h->d_K, nn));
     HIPCHK(dalloc(&h->d_dmax, 1));
     HIPCHK(dalloc(&h->d_counters, 3 * h->work.size() + 3));
+    return QCJK_OK;
+}
+
+static ClassPairArgs class_args(const qcjk_handle* h, const ClassPairWork& w, int variant)
+{
+    ClassPairArgs c{};
+    c.bra_off = h->coff[w.bra];
+    c.nbra = h->coff[w.bra + 1] - h->coff[w.bra];
+    c.ket_off = h->coff[w.ket];
+    c.nket = h->coff[w.ket + 1] - h->coff[w.ket];
+    c.same = w.bra == w.ket;
+    c.unit_start = w.d_start[variant];
+    c.nsurv = w.d_nsurv;
+    c.nunits = w.nunits[variant];
+    c.tile = kTile[variant];
+    return c;
+}
+
+// Prescale D, zero the accumulators and counters (start of every build).
+static int begin_build(qcjk_handle* h, const double* dD, hipStream_t s)
+{
+    const int n = h->n;
+    const long long nn = (long long)n * n;
+    const int eb = (int)std::min<long long>((nn + 255) / 256, 4096);
+    const int nctr = (int)(3 * h->work.size() + 3);
+    hipLaunchKernelGGL(k_zero_u64, dim3((nctr + 255) / 256), dim3(256), 0, s, h->d_counters, nctr);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, h->d_dmax, 1);
+    hipLaunchKernelGGL(k_prescale, dim3(eb), dim3(256), 0, s, dD, h->d_ns, n, h->d_Dp, h->d_Jt, h->d_Kt, h->d_dmax);
+    HIPCHK(hipGetLastError());
     return QCJK_OK;
 }
 
@@ -434,7 +472,8 @@ int qcjk_profile(qcjk_handle* h, int enable)
     for (size_t i = 0; i < h->work.size(); ++i) {
         const ClassKernelInfo* k = h->work[i].info;
         h->ptot[i].la = k->la; h->ptot[i].lb = k->lb; h->ptot[i].lc = k->lc; h->ptot[i].ld = k->ld;
-        h->ptot[i].wavefront_per_quartet = k->tile == 1 ? 1 : 0;
+        h->ptot[i].wavefront_per_quartet = h->work[i].variant;
+        h->ptot[i].alt_ms = -1.0;
     }
     return QCJK_OK;
 }
@@ -495,27 +534,18 @@ int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, 
         if (rc != QCJK_OK) return rc;
     }
     if (stats) HIPCHK(hipEventRecord(h->ev0, s));
-    hipLaunchKernelGGL(k_zero_u64, dim3((nctr + 255) / 256), dim3(256), 0, s, h->d_counters, nctr);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, h->d_dmax, 1);
-    hipLaunchKernelGGL(k_prescale, dim3(eb), dim3(256), 0, s, dD, h->d_ns, n, h->d_Dp, h->d_Jt, h->d_Kt, h->d_dmax);
-    HIPCHK(hipGetLastError());
+    {
+        const int rc = begin_build(h, dD, s);
+        if (rc != QCJK_OK) return rc;
+    }
     if (flags & (QCJK_J | QCJK_K)) {
         BuildArgs a = make_args(h, flags);
         for (size_t i = 0; i < h->work.size(); ++i) {
             const ClassPairWork& w = h->work[i];
-            ClassPairArgs c{};
-            c.bra_off = h->coff[w.bra];
-            c.nbra = h->coff[w.bra + 1] - h->coff[w.bra];
-            c.ket_off = h->coff[w.ket];
-            c.nket = h->coff[w.ket + 1] - h->coff[w.ket];
-            c.same = w.bra == w.ket;
-            c.unit_start = w.d_start;
-            c.nsurv = w.d_nsurv;
-            c.nunits = w.nunits;
-            c.tile = w.tile;
+            const ClassPairArgs c = class_args(h, w, w.variant);
             a.counters = h->d_counters + 3 * i;
             if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i], s));
-            HIPCHK(w.info->launch(a, c, s));
+            HIPCHK(w.launcher(w.variant)(a, c, s));
             if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i + 1], s));
         }
     }
@@ -623,6 +653,63 @@ int qcjk_schwarz(qcjk_handle* h, double* Q)
 }
 
 void qcjk_destroy(qcjk_handle* h) { free_handle(h); }
+
+int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant)
+{
+    if (!h || variant < 0 || variant > 1) return QCJK_ERR_ARG;
+    for (auto& w : h->work)
+        if (w.info->la == la && w.info->lb == lb && w.info->lc == lc && w.info->ld == ld) {
+            if (!w.has(variant)) return QCJK_ERR_UNSUPPORTED;
+            w.variant = variant;
+            return QCJK_OK;
+        }
+    return QCJK_OK;  // class absent from this basis: nothing to set
+}
+
+int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* out, int max, int* count)
+{
+    if (!h || !dD || reps < 1) return QCJK_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = begin_build(h, dD, s);
+    const BuildArgs a = make_args(h, QCJK_J | QCJK_K);
+    std::vector<qcjk_class_stat> res;
+    for (size_t i = 0; i < h->work.size() && rc == QCJK_OK; ++i) {
+        ClassPairWork& w = h->work[i];
+        double best[2] = {1e300, 1e300};
+        for (int v = 0; v < 2 && rc == QCJK_OK; ++v) {
+            if (!w.has(v)) continue;
+            const ClassPairArgs c = class_args(h, w, v);
+            for (int r = 0; r < reps + 1 && rc == QCJK_OK; ++r) {  // first launch is a warm-up
+                if (hipEventRecord(e0, s) != hipSuccess || w.launcher(v)(a, c, s) != hipSuccess ||
+                    hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = QCJK_ERR_HIP;
+                    break;
+                }
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = QCJK_ERR_HIP;
+                if (r > 0) best[v] = std::min(best[v], (double)ms);
+            }
+        }
+        w.variant = best[0] < best[1] ? 0 : 1;
+        qcjk_class_stat st{};
+        st.la = w.info->la; st.lb = w.info->lb; st.lc = w.info->lc; st.ld = w.info->ld;
+        st.wavefront_per_quartet = w.variant;
+        st.quartets = w.nquartets;
+        st.device_ms = best[w.variant];
+        st.alt_ms = best[1 - w.variant] < 1e299 ? best[1 - w.variant] : -1.0;
+        res.push_back(st);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != QCJK_OK) return rc;
+    if (count) *count = (int)res.size();
+    for (int i = 0; i < (int)res.size() && i < max && out; ++i) out[i] = res[i];
+    return QCJK_OK;
+}
 
 const char* qcjk_strerror(int code)
 {

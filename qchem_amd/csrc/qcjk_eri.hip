@@ -40,11 +40,12 @@ __device__ __forceinline__ double wave_sum(double v)
 // ---------------------------------------------------------------------------------------------
 // Lane-per-quartet kernel
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool BRA_MEM>
 __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
+    __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
     const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (unit >= C.nunits) return;  // uniform per wavefront
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
 
     double I[NAB * NCD];
     static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = 0.0; });
-    if (active) light_compute<LA, LB, LC, LD>(A, pi, pg, qi, qg, I);
+    if (active) light_compute<LA, LB, LC, LD, BRA_MEM>(A, pi, pg, qi, qg, I, sEab[BRA_MEM ? (threadIdx.x >> 6) : 0]);
 
     double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
     if (!active) s = 0.0;
@@ -216,12 +217,15 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Launchers + dispatch table
 // ---------------------------------------------------------------------------------------------
+// Bra Hermite tables go to LDS once they would cost more registers than the rest of the lane's state.
+constexpr bool bra_in_lds(int la, int lb) { return la + lb >= 3; }
+
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_light(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
     const long long blocks = (c.nunits + 3) / 4;
-    hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
+    hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
     return hipGetLastError();
 }
 
@@ -235,20 +239,27 @@ hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
     return hipGetLastError();
 }
 
-// Light if the Cartesian block fits comfortably in one lane's registers.
-constexpr bool is_light(int la, int lb, int lc, int ld)
+// Classes with a lane-per-quartet variant compiled (total angular momentum <= 4, block <= 81).
+constexpr bool has_light(int la, int lb, int lc, int ld)
 {
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= 36 && la + lb <= 2 && lc + ld <= 2;
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= 81 && la + lb + lc + ld <= 4;
+}
+// Default choice before tuning: lane per quartet where it stays out of scratch memory.
+constexpr bool default_light(int la, int lb, int lc, int ld)
+{
+    return has_light(la, lb, lc, ld) && !(la == 2 && lb == 1 && lc == 1) && !(la == 2 && lb == 2);
 }
 
 template <int LA, int LB, int LC, int LD>
 constexpr ClassKernelInfo make_info()
 {
-    if constexpr (is_light(LA, LB, LC, LD)) return {LA, LB, LC, LD, 64, &launch_light<LA, LB, LC, LD>};
-    else return {LA, LB, LC, LD, 1, &launch_heavy<LA, LB, LC, LD>};
+    ClassLauncher light = nullptr;
+    if constexpr (has_light(LA, LB, LC, LD)) light = &launch_light<LA, LB, LC, LD>;
+    return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1,
+            default_light(LA, LB, LC, LD) ? light : &launch_heavy<LA, LB, LC, LD>, light,
+            &launch_heavy<LA, LB, LC, LD>};
 }
 #define QCJK_CLASS(LA, LB, LC, LD) make_info<LA, LB, LC, LD>()
-#define QCJK_HEAVY(LA, LB, LC, LD) {LA, LB, LC, LD, 1, &launch_heavy<LA, LB, LC, LD>}
 
 // [bra class][ket class], bra >= ket; pair classes ss ps pp ds dp dd.
 static const ClassKernelInfo g_table[kNumPairClasses][kNumPairClasses] = {
@@ -261,24 +272,11 @@ static const ClassKernelInfo g_table[kNumPairClasses][kNumPairClasses] = {
     {QCJK_CLASS(2, 2, 0, 0), QCJK_CLASS(2, 2, 1, 0), QCJK_CLASS(2, 2, 1, 1), QCJK_CLASS(2, 2, 2, 0),
      QCJK_CLASS(2, 2, 2, 1), QCJK_CLASS(2, 2, 2, 2)},
 };
-static const ClassKernelInfo g_heavy_table[kNumPairClasses][kNumPairClasses] = {
-    {QCJK_HEAVY(0, 0, 0, 0)},
-    {QCJK_HEAVY(1, 0, 0, 0), QCJK_HEAVY(1, 0, 1, 0)},
-    {QCJK_HEAVY(1, 1, 0, 0), QCJK_HEAVY(1, 1, 1, 0), QCJK_HEAVY(1, 1, 1, 1)},
-    {QCJK_HEAVY(2, 0, 0, 0), QCJK_HEAVY(2, 0, 1, 0), QCJK_HEAVY(2, 0, 1, 1), QCJK_HEAVY(2, 0, 2, 0)},
-    {QCJK_HEAVY(2, 1, 0, 0), QCJK_HEAVY(2, 1, 1, 0), QCJK_HEAVY(2, 1, 1, 1), QCJK_HEAVY(2, 1, 2, 0),
-     QCJK_HEAVY(2, 1, 2, 1)},
-    {QCJK_HEAVY(2, 2, 0, 0), QCJK_HEAVY(2, 2, 1, 0), QCJK_HEAVY(2, 2, 1, 1), QCJK_HEAVY(2, 2, 2, 0),
-     QCJK_HEAVY(2, 2, 2, 1), QCJK_HEAVY(2, 2, 2, 2)},
-};
-
-static bool g_force_heavy = false;
-void set_force_heavy(bool on) { g_force_heavy = on; }
 
 const ClassKernelInfo* class_kernel(int bra_class, int ket_class)
 {
     if (bra_class < 0 || bra_class >= kNumPairClasses || ket_class < 0 || ket_class > bra_class) return nullptr;
-    return g_force_heavy ? &g_heavy_table[bra_class][ket_class] : &g_table[bra_class][ket_class];
+    return &g_table[bra_class][ket_class];
 }
 
 }  // namespace qcjk

@@ -51,15 +51,18 @@ struct ClassPairArgs {
 // Launch one class-pair kernel.  Returns hipError_t.
 typedef hipError_t (*ClassLauncher)(const BuildArgs&, const ClassPairArgs&, hipStream_t);
 
+// Kernel variants of one class: lane per quartet (tile 64, absent for large classes) and
+// wavefront per quartet (tile 1, always present).  tile/launch are the untuned default.
 struct ClassKernelInfo {
     int la, lb, lc, ld;
-    int tile;              // 64 = lane per quartet, 1 = wavefront per quartet
+    int tile;
     ClassLauncher launch;
+    ClassLauncher light;   // may be null
+    ClassLauncher heavy;
 };
 
 // Table of the 21 bra-class >= ket-class combinations (qcjk_eri.hip).
 const ClassKernelInfo* class_kernel(int bra_class, int ket_class);
-void set_force_heavy(bool on);  // route every class through the wavefront-per-quartet kernel
 
 // Setup kernels (qcjk_setup.hip).
 hipError_t launch_pair_setup(const int4* pair_shells /* {A,B,poffA,poffB} */, const int2* pair_np,

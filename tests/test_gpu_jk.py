@@ -101,9 +101,9 @@ def test_flags_j_only_k_only(require_gpu):
     J, K, _ = eng.build(D)
     Jo, Ko, _ = eng.build(D, flags=qa.QCJK_J)
     assert np.array_equal(Ko, np.zeros_like(Ko))
-    assert rel_err(Jo, J) < 1e-14
+    assert rel_err(Jo, J) < 1e-12  # FP64 atomics: accumulation order differs run to run
     Jo, Ko, _ = eng.build(D, flags=qa.QCJK_K)
-    assert rel_err(Ko, K) < 1e-14
+    assert rel_err(Ko, K) < 1e-12
 
 
 def test_one_electron_parity(require_gpu):
@@ -128,3 +128,36 @@ def test_n2_rhf_energy(require_gpu):
         res["energy"], ref["energy"], res["energy"] - ref["energy"], res["iterations"], ref["iterations"]))
     assert res["converged"] and ref["converged"]
     assert abs(res["energy"] - ref["energy"]) < 1e-8
+
+
+def test_every_variant_of_every_class(require_gpu):
+    """Both kernel variants of every class (where compiled) agree with the oracle."""
+    m = small_cluster(5, 51)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 51)
+    ob = oracle.Basis(m["shells"])
+    Jr, Kr, _ = ob.jk_direct(D)
+    classes = [(a, b, c, d) for a in range(3) for b in range(a + 1) for c in range(3) for d in range(c + 1)
+               if (a, b) >= (c, d) or a * 3 + b >= c * 3 + d]
+    for variant in (0, 1):
+        eng = qa.JKEngine(B)
+        for (a, b, c, d) in classes:
+            try:
+                eng.set_variant(a, b, c, d, variant)
+            except qa.EngineError as e:
+                assert e.code == 2 and variant == 0  # no lane-per-quartet kernel for this class
+        J, K, _ = eng.build(D)
+        assert rel_err(J, Jr) < TOL and rel_err(K, Kr) < TOL, variant
+
+
+def test_autotune_keeps_results(require_gpu):
+    import torch
+    m = small_cluster(6, 61)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 61)
+    eng = qa.JKEngine(B)
+    J0, K0, _ = eng.build(D)
+    rep = eng.autotune(torch.from_numpy(D).cuda(), reps=1)
+    assert rep and all(r["device_ms"] > 0 for r in rep)
+    J1, K1, _ = eng.build(D)
+    assert rel_err(J1, J0) < 1e-12 and rel_err(K1, K0) < 1e-12
