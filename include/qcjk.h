@@ -118,7 +118,7 @@ int qcjk_schwarz(qcjk_handle* h, double* Q);
  * the launches, elapsed device time and the algorithmic FLOPs of the quartets it computed. */
 typedef struct qcjk_class_stat {
     int32_t la, lb, lc, ld;
-    int32_t wavefront_per_quartet;  /* 1: LDS-staged wavefront kernel, 0: lane-per-quartet kernel */
+    int32_t wavefront_per_quartet;  /* kernel variant in use: 0 lane per quartet, 1 / 2 wavefront per quartet */
     int64_t launches;
     int64_t quartets;
     int64_t prim_quartets;
@@ -131,11 +131,12 @@ int qcjk_profile(qcjk_handle* h, int enable);  /* enable != 0 also resets the to
 int qcjk_class_report(qcjk_handle* h, qcjk_class_stat* out, int max, int* count);
 
 /* Kernel variant per angular-momentum class (no reference counterpart): 0 = lane per contracted
- * quartet (register resident; compiled for classes with L <= 4), 1 = wavefront per quartet (LDS
- * staged; every class).  qcjk_autotune times both variants of every class present in this basis on
- * the given DEVICE density (reps timed launches each, after one warm-up), keeps the faster one and
- * reports, per class, the chosen variant in wavefront_per_quartet, its time in device_ms and the
- * other variant's time in alt_ms (-1 when it has only one).  Results are cached by the caller
+ * quartet (register resident; compiled for classes with L <= 4), 1 = wavefront per quartet with
+ * LDS-staged Hermite transforms, 2 = wavefront per quartet with register-resident transforms (1 and
+ * 2 exist for every class).  qcjk_autotune times every variant of every class present in this
+ * basis on the given DEVICE density (reps timed launches each, after one warm-up), keeps the
+ * fastest and reports, per class, the chosen variant in wavefront_per_quartet, its time in
+ * device_ms and the best other variant's time in alt_ms.  Results are cached by the caller
  * (qchem_amd.tune) and re-applied with qcjk_set_variant. */
 int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant);
 int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* out, int max, int* count);

@@ -26,8 +26,10 @@ namespace {
 
 constexpr double kPruneTol = 1e-30;  // create-time: drop quartets whose Schwarz product is below this
 
-// Variants: 0 = lane per quartet (64 ket pairs per unit), 1 = wavefront per quartet (1 per unit).
-constexpr int kTile[2] = {64, 1};
+// Variants: 0 = lane per quartet (64 ket pairs per unit), 1 = wavefront per quartet with LDS
+// transforms, 2 = wavefront per quartet with register transforms (1 ket pair per unit).
+constexpr int kNumVariants = 3;
+constexpr int kTile[kNumVariants] = {64, 1, 1};
 
 struct ClassPairWork {
     int bra, ket;
@@ -39,8 +41,9 @@ struct ClassPairWork {
     int* d_nsurv = nullptr;
     double prim_flops = 0;     // per primitive quartet
     double digest_flops = 0;   // per contracted quartet
-    bool has(int v) const { return v == 1 || info->light != nullptr; }
-    ClassLauncher launcher(int v) const { return v == 0 ? info->light : info->heavy; }
+    bool has(int v) const { return v == 1 || v == 2 || (v == 0 && info->light != nullptr); }
+    ClassLauncher launcher(int v) const { return v == 0 ? info->light : (v == 1 ? info->heavy : info->heavy2); }
+    int unit_list(int v) const { return v == 0 ? 0 : 1; }
 };
 
 // Boys table F_j(0.1 k), k = 0..400, j = 0..23 (positive-term series, long double).
@@ -413,9 +416,9 @@ static ClassPairArgs class_args(const qcjk_handle* h, const ClassPairWork& w, in
     c.ket_off = h->coff[w.ket];
     c.nket = h->coff[w.ket + 1] - h->coff[w.ket];
     c.same = w.bra == w.ket;
-    c.unit_start = w.d_start[variant];
+    c.unit_start = w.d_start[w.unit_list(variant)];
     c.nsurv = w.d_nsurv;
-    c.nunits = w.nunits[variant];
+    c.nunits = w.nunits[w.unit_list(variant)];
     c.tile = kTile[variant];
     return c;
 }
@@ -656,7 +659,7 @@ void qcjk_destroy(qcjk_handle* h) { free_handle(h); }
 
 int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant)
 {
-    if (!h || variant < 0 || variant > 1) return QCJK_ERR_ARG;
+    if (!h || variant < 0 || variant >= kNumVariants) return QCJK_ERR_ARG;
     for (auto& w : h->work)
         if (w.info->la == la && w.info->lb == lb && w.info->lc == lc && w.info->ld == ld) {
             if (!w.has(variant)) return QCJK_ERR_UNSUPPORTED;
@@ -679,8 +682,8 @@ int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* o
     std::vector<qcjk_class_stat> res;
     for (size_t i = 0; i < h->work.size() && rc == QCJK_OK; ++i) {
         ClassPairWork& w = h->work[i];
-        double best[2] = {1e300, 1e300};
-        for (int v = 0; v < 2 && rc == QCJK_OK; ++v) {
+        double best[kNumVariants] = {1e300, 1e300, 1e300};
+        for (int v = 0; v < kNumVariants && rc == QCJK_OK; ++v) {
             if (!w.has(v)) continue;
             const ClassPairArgs c = class_args(h, w, v);
             for (int r = 0; r < reps + 1 && rc == QCJK_OK; ++r) {  // first launch is a warm-up
@@ -694,13 +697,19 @@ int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* o
                 if (r > 0) best[v] = std::min(best[v], (double)ms);
             }
         }
-        w.variant = best[0] < best[1] ? 0 : 1;
+        int bv = 0;
+        for (int v = 1; v < kNumVariants; ++v)
+            if (best[v] < best[bv]) bv = v;
+        double alt = 1e300;
+        for (int v = 0; v < kNumVariants; ++v)
+            if (v != bv) alt = std::min(alt, best[v]);
+        w.variant = bv;
         qcjk_class_stat st{};
         st.la = w.info->la; st.lb = w.info->lb; st.lc = w.info->lc; st.ld = w.info->ld;
         st.wavefront_per_quartet = w.variant;
         st.quartets = w.nquartets;
-        st.device_ms = best[w.variant];
-        st.alt_ms = best[1 - w.variant] < 1e299 ? best[1 - w.variant] : -1.0;
+        st.device_ms = best[bv];
+        st.alt_ms = alt < 1e299 ? alt : -1.0;
         res.push_back(st);
     }
     hipEventDestroy(e0);

@@ -16,6 +16,7 @@
 //   k_jk_heavy  wavefront per contracted quartet with LDS staging (mnd_heavy.h), for classes whose
 //               Cartesian block exceeds a lane's registers.
 #include "mnd_quartet.h"
+#include "mnd_heavy2.h"
 
 namespace qcjk {
 
@@ -215,6 +216,46 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
+{
+    using Dg = HeavyDigest<LA, LB, LC, LD>;
+    using S = HeavyShape<LA, LB, LC, LD>;
+    __shared__ H2Smem<LA, LB, LC, LD> sm;
+    const int lane = threadIdx.x;
+    heavy2_init<LA, LB, LC, LD>(sm, lane);
+    int t0 = 0, u0 = 0, v0 = 0;
+    if (lane < S::NHAB) hdec(lane, t0, u0, v0);
+    const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
+        const int P = find_unit_owner(C.unit_start, C.nbra, u);
+        const int Q = (int)(u - C.unit_start[P]);
+        const int4 pi = A.pinfo[C.bra_off + P];
+        const double4 pg = A.pgeo[C.bra_off + P];
+        const int4 qi = A.pinfo[C.ket_off + Q];
+        const double4 qg = A.pgeo[C.ket_off + Q];
+        if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
+        heavy2_compute<LA, LB, LC, LD>(A, sm, lane, t0, u0, v0, pi, pg, qi, qg);
+        const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+        for (int t = lane; t < Dg::NTASK; t += 64) {
+            int row, col;
+            double val;
+            if (Dg::task(t, sm.I, A.D, A.n, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
+                double* tgt = t < Dg::T2 ? A.Jt : A.Kt;
+                unsafeAtomicAdd(&tgt[row * A.n + col], val);
+            }
+        }
+        if (A.counters && lane == 0) {
+            atomicAdd(&A.counters[0], 1ull);
+            atomicAdd(&A.counters[2], (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
+        }
+        __syncthreads();  // digest reads of sm.I finish before the next quartet clears it
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Launchers + dispatch table
 // ---------------------------------------------------------------------------------------------
 // Bra Hermite tables go to LDS once they would cost more registers than the rest of the lane's state.
@@ -239,6 +280,16 @@ hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
     return hipGetLastError();
 }
 
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    if (c.nunits <= 0) return hipSuccess;
+    const long long cap = 256LL * 64;
+    const long long blocks = c.nunits < cap ? c.nunits : cap;
+    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    return hipGetLastError();
+}
+
 // Classes with a lane-per-quartet variant compiled (total angular momentum <= 4, block <= 81).
 constexpr bool has_light(int la, int lb, int lc, int ld)
 {
@@ -257,7 +308,7 @@ constexpr ClassKernelInfo make_info()
     if constexpr (has_light(LA, LB, LC, LD)) light = &launch_light<LA, LB, LC, LD>;
     return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1,
             default_light(LA, LB, LC, LD) ? light : &launch_heavy<LA, LB, LC, LD>, light,
-            &launch_heavy<LA, LB, LC, LD>};
+            &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>};
 }
 #define QCJK_CLASS(LA, LB, LC, LD) make_info<LA, LB, LC, LD>()
 
