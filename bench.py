@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 METRIC = "Fock-build s & ERI quartets/s; SCF time-to-1e-6 Eh: mixed+learned vs FP64 GPU"
+VARIANT_KERNEL = {0: "light", 1: "heavy", 2: "heavy2", 3: "lightc"}
 
 
 def cpu_baseline(mol, D, budget_s=15.0, threads=None):
@@ -56,8 +57,21 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-tune", action="store_true", help="skip the per-class kernel-variant autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="use the untuned default kernel variant per class")
+    ap.add_argument("--retune", action="store_true", help="re-run the per-class kernel-variant autotuner")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a copy kernel of known size (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 64 * 1024 * 1024  # 512 MiB of FP64 read, 512 MiB written per launch
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy": {"bytes_read": n * 8, "bytes_written": n * 8, "launches": 3}}))
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; the multi-GPU path is design-only (--gpus %d requested)"
               % args.gpus, file=sys.stderr)
@@ -84,10 +98,16 @@ def main():
 
     flags = qa.QCJK_J | qa.QCJK_K
     tuning = None
-    if not args.no_tune:  # untimed setup: pick the faster kernel variant per class on this workload
-        from qchem_amd import tune
-        tuning = tune.tune(eng, D, reps=2, cache=os.path.join(ROOT, "gpurun_out", "tuning_gfx950.json"),
-                           workload="random-cluster-16")
+    from qchem_amd import tune
+    workload_key = "random-cluster-16"
+    if args.retune:  # untimed setup: time every kernel variant per class on this workload
+        rep = tune.tune(eng, D, reps=2, cache=os.path.join(ROOT, "gpurun_out", "tuning_gfx950.json"),
+                        workload=workload_key)
+        tuning = {"source": "autotuned now", "classes": {"%d%d%d%d" % (r["la"], r["lb"], r["lc"], r["ld"]): [
+            r["wavefront_per_quartet"], round(r["device_ms"], 3), round(r["alt_ms"], 3)] for r in rep}}
+    elif not args.no_tune:  # the committed per-class choices for this workload on gfx950
+        table = tune.apply(eng, workload=workload_key)
+        tuning = {"source": "qchem_amd/tuning/gfx950.json", "classes": table}
     st = eng.build_device(D, J, K, flags, stream=stream, stats=True)  # counts + first build
     for _ in range(args.warmup):
         eng.build_device(D, J, K, flags, stream=stream)
@@ -114,7 +134,7 @@ def main():
     total_flops = sum(r["flops"] for r in rep) / max(1, args.steps)
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": "k_jk_%s<%d,%d,%d,%d>" % ("heavy" if dom["wavefront_per_quartet"] else "light",
+            "kernel": "k_jk_%s<%d,%d,%d,%d>" % (VARIANT_KERNEL[dom["wavefront_per_quartet"]],
                                                 dom["la"], dom["lb"], dom["lc"], dom["ld"]),
             "kernel_ms_per_launch": per_launch_s * 1e3, "kernel_flops_per_launch": per_launch_flops,
             "build_tflops": total_flops / (ms / 1e3) / 1e12}
@@ -128,9 +148,7 @@ def main():
                       "n_basis": n, "quartets_per_build": quartets, "fock_build_s": ms / 1e3,
                       "parallelism": "single GPU"},
            "roofline": roof, "cpu_baseline": cpu,
-           "tuning": None if tuning is None else {"%d%d%d%d" % (r["la"], r["lb"], r["lc"], r["ld"]):
-                                                  [r["wavefront_per_quartet"], round(r["device_ms"], 3),
-                                                   round(r["alt_ms"], 3)] for r in tuning},
+           "tuning": tuning,
            "classes": [{k: r[k] for k in ("la", "lb", "lc", "ld", "wavefront_per_quartet", "quartets")} |
                        {"ms": r["device_ms"] / max(1, r["launches"]),
                         "tflops": r["flops"] / max(1e-12, r["device_ms"] / 1e3) / 1e12} for r in rep]}

@@ -306,6 +306,135 @@ QC_HD void prim_quartet(const HermiteAxis<LA, LB, R>* eab, const HermiteAxis<LC,
     });
 }
 
+// Contraction restricted to ket first component CI (a "chunk"): I[(a*NB+b)*ND+d] for c = CI only.
+template <int LA, int LB, int LC, int LD, int CI, typename R>
+QC_HD void prim_quartet_chunk(const HermiteAxis<LA, LB, R>* eab, const HermiteAxis<LC, LD, R>* ecd,
+                              const R* Rt, R lam, R* I)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), ND = ncart(LD);
+    constexpr int LAB = LA + LB;
+    constexpr int NHAB = nherm(LAB);
+    constexpr int cx = comp(LC, CI, 0), cy = comp(LC, CI, 1), cz = comp(LC, CI, 2);
+    static_for<0, ND>([&](auto dd) {
+        constexpr int d = decltype(dd)::value;
+        constexpr int dx = comp(LD, d, 0), dy = comp(LD, d, 1), dz = comp(LD, d, 2);
+        R x[NHAB];
+        static_for<0, LAB + 1>([&](auto tt) {
+            constexpr int t = decltype(tt)::value;
+            static_for<0, LAB + 1 - t>([&](auto uu) {
+                constexpr int u = decltype(uu)::value;
+                static_for<0, LAB + 1 - t - u>([&](auto vv) {
+                    constexpr int v = decltype(vv)::value;
+                    R acc = R(0);
+                    static_for<0, cx + dx + 1>([&](auto t2) {
+                        constexpr int tau = decltype(t2)::value;
+                        static_for<0, cy + dy + 1>([&](auto u2) {
+                            constexpr int nu = decltype(u2)::value;
+                            R exy = ecd[0].e[cx][dx][tau] * ecd[1].e[cy][dy][nu];
+                            static_for<0, cz + dz + 1>([&](auto v2) {
+                                constexpr int phi = decltype(v2)::value;
+                                R term = exy * ecd[2].e[cz][dz][phi] * Rt[hidx(t + tau, u + nu, v + phi)];
+                                if constexpr (((tau + nu + phi) & 1) != 0) acc -= term;
+                                else acc += term;
+                            });
+                        });
+                    });
+                    x[hidx(t, u, v)] = acc;
+                });
+            });
+        });
+        static_for<0, NA>([&](auto aa) {
+            constexpr int a = decltype(aa)::value;
+            constexpr int ax = comp(LA, a, 0), ay = comp(LA, a, 1), az = comp(LA, a, 2);
+            static_for<0, NB>([&](auto bb) {
+                constexpr int b = decltype(bb)::value;
+                constexpr int bx = comp(LB, b, 0), by = comp(LB, b, 1), bz = comp(LB, b, 2);
+                R acc = R(0);
+                static_for<0, ax + bx + 1>([&](auto tt) {
+                    constexpr int t = decltype(tt)::value;
+                    static_for<0, ay + by + 1>([&](auto uu) {
+                        constexpr int u = decltype(uu)::value;
+                        R exy = eab[0].e[ax][bx][t] * eab[1].e[ay][by][u];
+                        static_for<0, az + bz + 1>([&](auto vv) {
+                            constexpr int v = decltype(vv)::value;
+                            acc += exy * eab[2].e[az][bz][v] * x[hidx(t, u, v)];
+                        });
+                    });
+                });
+                I[(a * NB + b) * ND + d] += lam * acc;
+            });
+        });
+    });
+}
+
+// Same contraction, two phases: the ket transform x[cd][tuv] for every ket component pair first
+// (R is dead afterwards), then the bra transform one bra component pair at a time.  Keeps the bra
+// Hermite products short-lived; used where NCD * nherm(Lab) is small.
+template <int LA, int LB, int LC, int LD, typename R>
+QC_HD void prim_quartet_two_phase(const HermiteAxis<LA, LB, R>* eab, const HermiteAxis<LC, LD, R>* ecd,
+                                  const R* Rt, R lam, R* I)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int LAB = LA + LB;
+    constexpr int NHAB = nherm(LAB);
+    constexpr int NCD = NC * ND;
+    R x[NCD][NHAB];
+    static_for<0, NC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        static_for<0, ND>([&](auto dd) {
+            constexpr int d = decltype(dd)::value;
+            constexpr int cx = comp(LC, c, 0), cy = comp(LC, c, 1), cz = comp(LC, c, 2);
+            constexpr int dx = comp(LD, d, 0), dy = comp(LD, d, 1), dz = comp(LD, d, 2);
+            static_for<0, LAB + 1>([&](auto tt) {
+                constexpr int t = decltype(tt)::value;
+                static_for<0, LAB + 1 - t>([&](auto uu) {
+                    constexpr int u = decltype(uu)::value;
+                    static_for<0, LAB + 1 - t - u>([&](auto vv) {
+                        constexpr int v = decltype(vv)::value;
+                        R acc = R(0);
+                        static_for<0, cx + dx + 1>([&](auto t2) {
+                            constexpr int tau = decltype(t2)::value;
+                            static_for<0, cy + dy + 1>([&](auto u2) {
+                                constexpr int nu = decltype(u2)::value;
+                                R exy = ecd[0].e[cx][dx][tau] * ecd[1].e[cy][dy][nu];
+                                static_for<0, cz + dz + 1>([&](auto v2) {
+                                    constexpr int phi = decltype(v2)::value;
+                                    R term = exy * ecd[2].e[cz][dz][phi] * Rt[hidx(t + tau, u + nu, v + phi)];
+                                    if constexpr (((tau + nu + phi) & 1) != 0) acc -= term;
+                                    else acc += term;
+                                });
+                            });
+                        });
+                        x[c * ND + d][hidx(t, u, v)] = lam * acc;
+                    });
+                });
+            });
+        });
+    });
+    static_for<0, NA>([&](auto aa) {
+        constexpr int a = decltype(aa)::value;
+        constexpr int ax = comp(LA, a, 0), ay = comp(LA, a, 1), az = comp(LA, a, 2);
+        static_for<0, NB>([&](auto bb) {
+            constexpr int b = decltype(bb)::value;
+            constexpr int bx = comp(LB, b, 0), by = comp(LB, b, 1), bz = comp(LB, b, 2);
+            static_for<0, ax + bx + 1>([&](auto tt) {
+                constexpr int t = decltype(tt)::value;
+                static_for<0, ay + by + 1>([&](auto uu) {
+                    constexpr int u = decltype(uu)::value;
+                    const R exy = eab[0].e[ax][bx][t] * eab[1].e[ay][by][u];
+                    static_for<0, az + bz + 1>([&](auto vv) {
+                        constexpr int v = decltype(vv)::value;
+                        const R e = exy * eab[2].e[az][bz][v];
+                        static_for<0, NCD>([&](auto k) {
+                            I[(a * NB + b) * NCD + decltype(k)::value] += e * x[decltype(k)::value][hidx(t, u, v)];
+                        });
+                    });
+                });
+            });
+        });
+    });
+}
+
 // Algorithmic FLOPs of one primitive quartet (the metric's per-unit formula):
 //   40 + 2(L+1) + 3 C(L+4,4) + 10 + 2 nh(Lab) nh(Lcd) nc(lc)nc(ld) + 2 nc(la)nc(lb) nh(Lab) nc(lc)nc(ld)
 constexpr double prim_flops(int la, int lb, int lc, int ld)

@@ -27,9 +27,10 @@ namespace {
 constexpr double kPruneTol = 1e-30;  // create-time: drop quartets whose Schwarz product is below this
 
 // Variants: 0 = lane per quartet (64 ket pairs per unit), 1 = wavefront per quartet with LDS
-// transforms, 2 = wavefront per quartet with register transforms (1 ket pair per unit).
-constexpr int kNumVariants = 3;
-constexpr int kTile[kNumVariants] = {64, 1, 1};
+// transforms, 2 = wavefront per quartet with register transforms (1 ket pair per unit),
+// 3 = lane per quartet chunked over the ket's first component (64 per unit).
+constexpr int kNumVariants = 4;
+constexpr int kTile[kNumVariants] = {64, 1, 1, 64};
 
 struct ClassPairWork {
     int bra, ket;
@@ -41,9 +42,15 @@ struct ClassPairWork {
     int* d_nsurv = nullptr;
     double prim_flops = 0;     // per primitive quartet
     double digest_flops = 0;   // per contracted quartet
-    bool has(int v) const { return v == 1 || v == 2 || (v == 0 && info->light != nullptr); }
-    ClassLauncher launcher(int v) const { return v == 0 ? info->light : (v == 1 ? info->heavy : info->heavy2); }
-    int unit_list(int v) const { return v == 0 ? 0 : 1; }
+    bool has(int v) const
+    {
+        return v == 1 || v == 2 || (v == 0 && info->light != nullptr) || (v == 3 && info->lightc != nullptr);
+    }
+    ClassLauncher launcher(int v) const
+    {
+        return v == 0 ? info->light : (v == 1 ? info->heavy : (v == 2 ? info->heavy2 : info->lightc));
+    }
+    int unit_list(int v) const { return (v == 0 || v == 3) ? 0 : 1; }
 };
 
 // Boys table F_j(0.1 k), k = 0..400, j = 0..23 (positive-term series, long double).
@@ -363,7 +370,7 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
             w.ket = kc;
             w.info = class_kernel(bc, kc);
             if (!w.info) return QCJK_ERR_UNSUPPORTED;
-            w.variant = (!force_heavy && w.info->tile == 64 && w.info->light) ? 0 : 1;
+            w.variant = force_heavy ? 1 : ((w.info->tile == 64 && w.info->light) ? 0 : 2);
             w.prim_flops = prim_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             w.digest_flops = digest_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             std::vector<long long> start0(nb + 1, 0), start1(nb + 1, 0);
@@ -682,7 +689,7 @@ int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* o
     std::vector<qcjk_class_stat> res;
     for (size_t i = 0; i < h->work.size() && rc == QCJK_OK; ++i) {
         ClassPairWork& w = h->work[i];
-        double best[kNumVariants] = {1e300, 1e300, 1e300};
+        double best[kNumVariants] = {1e300, 1e300, 1e300, 1e300};
         for (int v = 0; v < kNumVariants && rc == QCJK_OK; ++v) {
             if (!w.has(v)) continue;
             const ClassPairArgs c = class_args(h, w, v);

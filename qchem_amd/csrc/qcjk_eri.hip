@@ -178,6 +178,137 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Lane-per-quartet kernel, chunked over the ket's first component: the primitive loop runs once
+// per chunk (R recomputed), so only NAB*ND accumulators are live.  J_ab, K_ad, K_bd accumulate
+// across chunks; J_cd, K_ac, K_bc of a chunk are final when it ends.
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, bool BRA_MEM>
+__global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB;
+    __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
+    const int lane = threadIdx.x & 63;
+    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (unit >= C.nunits) return;  // uniform per wavefront
+    const int P = find_unit_owner(C.unit_start, C.nbra, unit);
+    const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
+    bool active = q < C.nsurv[P];
+    const int Q = active ? q : 0;
+    const int4 pi = A.pinfo[C.bra_off + P];
+    const double4 pg = A.pgeo[C.bra_off + P];
+    const int4 qi = A.pinfo[C.ket_off + Q];
+    const double4 qg = A.pgeo[C.ket_off + Q];
+    if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
+    double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+    if (!active) s = 0.0;
+    const int n = A.n;
+    const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
+    const double* __restrict__ D = A.D;
+    const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    double jab[NAB], kad[NA * ND], kbd[NB * ND];
+    static_for<0, NAB>([&](auto i) { jab[decltype(i)::value] = 0.0; });
+    static_for<0, NA * ND>([&](auto i) { kad[decltype(i)::value] = 0.0; });
+    static_for<0, NB * ND>([&](auto i) { kbd[decltype(i)::value] = 0.0; });
+    static_for<0, NC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        double I[NAB * ND];
+        static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = 0.0; });
+        if (active)
+            light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM>(A, pi, pg, qi, qg, I,
+                                                             sEab[BRA_MEM ? (threadIdx.x >> 6) : 0]);
+        if (active && doJ) {
+            static_for<0, ND>([&](auto dd) {
+                constexpr int d = decltype(dd)::value;
+                const double dcd = D[(oc + c) * n + od + d];
+                double v = 0.0;
+                static_for<0, NAB>([&](auto ab) {
+                    constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+                    jab[decltype(ab)::value] += I[decltype(ab)::value * ND + d] * dcd;
+                    v += I[decltype(ab)::value * ND + d] * D[(oa + a) * n + ob + b];
+                });
+                unsafeAtomicAdd(&A.Jt[(oc + c) * n + od + d], 2.0 * s * v);
+            });
+        }
+        if (active && doK) {
+            static_for<0, NA>([&](auto aa) {  // Kt_ac += s sum_bd I D_bd
+                constexpr int a = decltype(aa)::value;
+                double v = 0.0;
+                static_for<0, NB>([&](auto bb) {
+                    constexpr int b = decltype(bb)::value;
+                    static_for<0, ND>([&](auto dd) {
+                        constexpr int d = decltype(dd)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + od + d];
+                    });
+                });
+                unsafeAtomicAdd(&A.Kt[(oa + a) * n + oc + c], s * v);
+            });
+            static_for<0, NB>([&](auto bb) {  // Kt_bc += s sum_ad I D_ad
+                constexpr int b = decltype(bb)::value;
+                double v = 0.0;
+                static_for<0, NA>([&](auto aa) {
+                    constexpr int a = decltype(aa)::value;
+                    static_for<0, ND>([&](auto dd) {
+                        constexpr int d = decltype(dd)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + od + d];
+                    });
+                });
+                unsafeAtomicAdd(&A.Kt[(ob + b) * n + oc + c], s * v);
+            });
+            static_for<0, NA>([&](auto aa) {  // Kt_ad += s sum_bc I D_bc (this chunk's c)
+                constexpr int a = decltype(aa)::value;
+                static_for<0, ND>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    double v = 0.0;
+                    static_for<0, NB>([&](auto bb) {
+                        constexpr int b = decltype(bb)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + oc + c];
+                    });
+                    kad[a * ND + d] += v;
+                });
+            });
+            static_for<0, NB>([&](auto bb) {  // Kt_bd += s sum_ac I D_ac (this chunk's c)
+                constexpr int b = decltype(bb)::value;
+                static_for<0, ND>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    double v = 0.0;
+                    static_for<0, NA>([&](auto aa) {
+                        constexpr int a = decltype(aa)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + oc + c];
+                    });
+                    kbd[b * ND + d] += v;
+                });
+            });
+        }
+    });
+    if (doJ) {
+        static_for<0, NAB>([&](auto ab) {
+            constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+            const double v = wave_sum(2.0 * s * jab[decltype(ab)::value]);
+            if (lane == 0) unsafeAtomicAdd(&A.Jt[(oa + a) * n + ob + b], v);
+        });
+    }
+    if (doK && active) {
+        static_for<0, NA * ND>([&](auto x) {
+            constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
+            unsafeAtomicAdd(&A.Kt[(oa + a) * n + od + d], s * kad[decltype(x)::value]);
+        });
+        static_for<0, NB * ND>([&](auto x) {
+            constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
+            unsafeAtomicAdd(&A.Kt[(ob + b) * n + od + d], s * kbd[decltype(x)::value]);
+        });
+    }
+    if (A.counters) {
+        const unsigned long long m = __ballot(active);
+        if (lane == 0 && m) {
+            const int cnt = __popcll(m);
+            atomicAdd(&A.counters[0], (unsigned long long)cnt);
+            atomicAdd(&A.counters[2], (unsigned long long)cnt * (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel (grid-stride over quartets)
 // ---------------------------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD>
@@ -271,6 +402,15 @@ hipError_t launch_light(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
 }
 
 template <int LA, int LB, int LC, int LD>
+hipError_t launch_lightc(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    if (c.nunits <= 0) return hipSuccess;
+    const long long blocks = (c.nunits + 3) / 4;
+    hipLaunchKernelGGL((k_jk_lightc<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
@@ -295,20 +435,22 @@ constexpr bool has_light(int la, int lb, int lc, int ld)
 {
     return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= 81 && la + lb + lc + ld <= 4;
 }
-// Default choice before tuning: lane per quartet where it stays out of scratch memory.
+// Default choice before tuning (measured on the random-cluster workload): lane per quartet for
+// L <= 4 except (dd|ss), the register-transform wavefront kernel otherwise.
 constexpr bool default_light(int la, int lb, int lc, int ld)
 {
-    return has_light(la, lb, lc, ld) && !(la == 2 && lb == 1 && lc == 1) && !(la == 2 && lb == 2);
+    return has_light(la, lb, lc, ld) && !(la == 2 && lb == 2);
 }
 
 template <int LA, int LB, int LC, int LD>
 constexpr ClassKernelInfo make_info()
 {
-    ClassLauncher light = nullptr;
+    ClassLauncher light = nullptr, lightc = nullptr;
     if constexpr (has_light(LA, LB, LC, LD)) light = &launch_light<LA, LB, LC, LD>;
+    if constexpr (has_light(LA, LB, LC, LD) && LC > 0) lightc = &launch_lightc<LA, LB, LC, LD>;
     return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1,
-            default_light(LA, LB, LC, LD) ? light : &launch_heavy<LA, LB, LC, LD>, light,
-            &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>};
+            default_light(LA, LB, LC, LD) ? light : &launch_heavy2<LA, LB, LC, LD>, light,
+            &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>, lightc};
 }
 #define QCJK_CLASS(LA, LB, LC, LD) make_info<LA, LB, LC, LD>()
 
