@@ -51,12 +51,45 @@ def cpu_baseline(mol, D, budget_s=15.0, threads=None):
                       "(%d contracted quartets, %.1f s)" % (limit2, npairs, cnt, dt)}
 
 
+def time_to(e_hist, t_hist, e_ref, tol=1e-6):
+    for e, t in zip(e_hist, t_hist):
+        if abs(e - e_ref) < tol:
+            return t
+    return None
+
+
+def scf_gpu(qa, mol, **kw):
+    """FP64 GPU RHF (C++ driver through the HF faces) converged tightly; time-to-1e-6 against itself."""
+    B = qa.Basis(mol["shells"])
+    res = qa.rhf(B, mol["Z"], mol["R"], e_tol=1e-10, fd_tol=1e-7, **kw)
+    return {"energy": res["energy"], "iterations": res["iterations"], "converged": res["converged"],
+            "time_s": res["t_total"], "jk_s": res["t_jk"], "diag_s": res["t_diag"],
+            "time_to_1e-6_s": time_to(res["energy_history"], res["time_history"], res["energy"])}
+
+
+def scf_cpu_reference(mol, e_conv):
+    """The reference algorithm on the CPU: ERI4 materialised once, MatMul + DIIS per iteration."""
+    import oracle
+    import oracle.scf as oscf
+
+    t0 = time.perf_counter()
+    ob = oracle.Basis(mol["shells"])
+    e4 = oracle.ERI4(ob)
+    t_eri4 = time.perf_counter() - t0
+    res = oscf.rhf(ob, mol["Z"], mol["R"], e4.jk, e_tol=1e-10, fd_tol=1e-7)
+    t1 = time_to(res["history"], res["times"], e_conv)
+    return {"energy": res["energy"], "iterations": res["iterations"], "cores": 1,
+            "eri4_build_s": t_eri4, "time_to_1e-6_s": None if t1 is None else t_eri4 + t1,
+            "delta_e_vs_gpu": res["energy"] - e_conv}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF time-to-solution section")
     ap.add_argument("--no-tune", action="store_true", help="use the untuned default kernel variant per class")
     ap.add_argument("--retune", action="store_true", help="re-run the per-class kernel-variant autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -140,6 +173,12 @@ def main():
             "build_tflops": total_flops / (ms / 1e3) / 1e12}
 
     cpu = None if args.no_cpu else cpu_baseline(mol, Dh)
+    scf = None
+    if not args.no_scf:
+        n2 = S.n2_even_tempered()
+        scf = {"n2_even_tempered_fp64_gpu": scf_gpu(qa, n2), "random_cluster_fp64_gpu": scf_gpu(qa, mol)}
+        if cpu is not None:
+            cpu["scf_n2_reference_algorithm"] = scf_cpu_reference(n2, scf["n2_even_tempered_fp64_gpu"]["energy"])
     out = {"metric": METRIC, "value": value, "unit": "quartets/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded random cluster, random PSD density)",
@@ -147,7 +186,7 @@ def main():
                                   % (mol["name"], n, quartets),
                       "n_basis": n, "quartets_per_build": quartets, "fock_build_s": ms / 1e3,
                       "parallelism": "single GPU"},
-           "roofline": roof, "cpu_baseline": cpu,
+           "roofline": roof, "cpu_baseline": cpu, "scf": scf,
            "tuning": tuning,
            "classes": [{k: r[k] for k in ("la", "lb", "lc", "ld", "wavefront_per_quartet", "quartets")} |
                        {"ms": r["device_ms"] / max(1, r["launches"]),

@@ -50,6 +50,12 @@ def lib():
         L.or_jk_direct.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_int, ctypes.c_longlong]
         L.or_jk_direct.restype = ctypes.c_longlong
         L.or_schwarz.argtypes = [ctypes.c_void_p, dp]
+        L.or_eri4_create.argtypes = [ctypes.c_void_p]
+        L.or_eri4_create.restype = ctypes.c_void_p
+        L.or_eri4_build_seconds.argtypes = [ctypes.c_void_p]
+        L.or_eri4_build_seconds.restype = ctypes.c_double
+        L.or_eri4_matmul.argtypes = [ctypes.c_void_p, dp, dp, dp]
+        L.or_eri4_destroy.argtypes = [ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -164,3 +170,26 @@ class Basis:
         Q = np.zeros((ns, ns))
         lib().or_schwarz(self.h, _dp(Q))
         return Q
+
+
+class ERI4:
+    """The reference algorithm's cached J4/K4 tensors: built once, contracted per SCF iteration."""
+
+    def __init__(self, basis):
+        self.basis = basis
+        self.h = lib().or_eri4_create(basis.h)
+        if not self.h:
+            raise RuntimeError("ERI4: primitive count too large for the reference algorithm's R cache")
+        self.build_seconds = lib().or_eri4_build_seconds(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_eri4_destroy(self.h)
+            self.h = None
+
+    def jk(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        n = self.basis.nbf
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        lib().or_eri4_matmul(self.h, _dp(D), _dp(J), _dp(K))
+        return J, K

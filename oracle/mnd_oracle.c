@@ -664,6 +664,86 @@ int or_jk_refalgo(void* h, const double* D, double* J, double* K, double* times)
     return 0;
 }
 
+/* The reference keeps the ERI4 tensors in a process-wide cache after the first build
+ * (DB_Cache_RAM.C:323-345) and only re-runs ERI4::MatMul per SCF iteration.  This context does the
+ * same: or_eri4_create materialises J4/K4 once (timed), or_eri4_matmul contracts per call. */
+typedef struct {
+    int n;
+    size_t npair;
+    double *J4, *K4;
+    double build_s;
+} or_eri4;
+
+void* or_eri4_create(void* h)
+{
+    or_basis* B = (or_basis*)h;
+    if (B->nprim_tot > 200) return NULL;
+    int n = B->nbf;
+    or_eri4* E = (or_eri4*)calloc(1, sizeof(or_eri4));
+    E->n = n;
+    E->npair = (size_t)n * (n + 1) / 2;
+    E->J4 = (double*)malloc(E->npair * E->npair * sizeof(double));
+    E->K4 = (double*)calloc(E->npair * E->npair, sizeof(double));
+    or_refctx X;
+    X.B = B;
+    X.nomega = B->nprim_tot * B->nprim_tot;
+    X.omega = (or_omega**)calloc(X.nomega, sizeof(or_omega*));
+    X.rcache = (double**)calloc((size_t)X.nomega * X.nomega, sizeof(double*));
+    X.rbytes = 0;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int ia = 0; ia < n; ia++)
+        for (int ib = ia; ib < n; ib++) {
+            double* blk = E->J4 + tri(ia, ib) * E->npair;
+            for (int ic = 0; ic < n; ic++)
+                for (int id = ic; id < n; id++) blk[tri(ic, id)] = fourc(&X, ia, ib, ic, id);
+        }
+    for (int ia = 0; ia < n; ia++)
+        for (int ib = 0; ib < n; ib++)
+            for (int ic = ia; ic < n; ic++) {
+                double* blk = E->K4 + tri(ia, ic) * E->npair;
+                for (int id = 0; id < n; id++) {
+                    double v = fourc(&X, ia, ib, ic, id);
+                    if (ib == id) blk[tri(ib, id)] = v;
+                    else blk[tri(ib, id)] += 0.5 * v;
+                }
+            }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    E->build_s = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    for (size_t i = 0; i < (size_t)X.nomega * X.nomega; ++i) free(X.rcache[i]);
+    for (int i = 0; i < X.nomega; ++i) free(X.omega[i]);
+    free(X.rcache); free(X.omega);
+    return E;
+}
+
+double or_eri4_build_seconds(void* e) { return ((or_eri4*)e)->build_s; }
+
+void or_eri4_matmul(void* e, const double* D, double* J, double* K)
+{
+    or_eri4* E = (or_eri4*)e;
+    int n = E->n;
+    for (int ia = 0; ia < n; ia++)
+        for (int ib = ia; ib < n; ib++) {
+            const double* jb = E->J4 + tri(ia, ib) * E->npair;
+            const double* kb = E->K4 + tri(ia, ib) * E->npair;
+            double sj = 0, sk = 0;
+            for (int c = 0; c < n; ++c)
+                for (int d = 0; d < n; ++d) {
+                    sj += jb[tri(c, d)] * D[c * n + d];
+                    sk += kb[tri(c, d)] * D[c * n + d];
+                }
+            J[ia * n + ib] = J[ib * n + ia] = sj;
+            K[ia * n + ib] = K[ib * n + ia] = sk;
+        }
+}
+
+void or_eri4_destroy(void* e)
+{
+    or_eri4* E = (or_eri4*)e;
+    if (!E) return;
+    free(E->J4); free(E->K4); free(E);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Direct 8-fold J/K (same mathematics, OpenMP over bra shell pairs)                           */
 /* ------------------------------------------------------------------------------------------ */

@@ -50,17 +50,21 @@ def rhf(basis, Z, R, jk, charge=0, max_iter=100, e_tol=1e-10, fd_tol=1e-7, nproj
     Fs, Es = [], []
     E_old = None
     hist = []
+    times = []
+    import time as _time
+    t_start = _time.perf_counter()
     for it in range(1, max_iter + 1):
         J, K = jk(D)
         F = h + J - 0.5 * K
         E = np.sum(D * h) + 0.5 * np.sum(D * J) - 0.25 * np.sum(D * K) + enuc
         hist.append(E)
+        times.append(_time.perf_counter() - t_start)
         Fp = X.T @ F @ X
         Dp = XtS @ D @ XtS.T
         Ep = Fp @ Dp - Dp @ Fp
         fd = np.linalg.norm(Ep)
         if E_old is not None and abs(E - E_old) < e_tol and fd < fd_tol:
-            return dict(energy=E, D=D, iterations=it, history=hist, converged=True)
+            return dict(energy=E, D=D, iterations=it, history=hist, times=times, converged=True)
         E_old = E
         Fx = Fp
         if fd <= fd_max:
@@ -88,4 +92,4 @@ def rhf(basis, Z, R, jk, charge=0, max_iter=100, e_tol=1e-10, fd_tol=1e-7, nproj
                     Fx = sum(ci * f for ci, f in zip(c, Fs))
                 break
         D = _density(X, Fx, nocc)[0]
-    return dict(energy=E, D=D, iterations=max_iter, history=hist, converged=False)
+    return dict(energy=E, D=D, iterations=max_iter, history=hist, times=times, converged=False)

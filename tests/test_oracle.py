@@ -144,7 +144,8 @@ def test_jk_routes_vs_brute_force():
     Kb = np.einsum("abcd,bd->ac", G, D)
     J1, K1, _ = ob.jk_refalgo(D)
     J2, K2, _ = ob.jk_direct(D)
-    for J, K in ((J1, K1), (J2, K2)):
+    J3, K3 = oracle.ERI4(ob).jk(D)  # the reference's materialised ERI4 + MatMul route
+    for J, K in ((J1, K1), (J2, K2), (J3, K3)):
         assert np.abs(J - Jb).max() < 1e-12 * np.abs(Jb).max()
         assert np.abs(K - Kb).max() < 1e-12 * np.abs(Kb).max()
     # a plausible packing mistake (K built with the J index pairing) must fail
