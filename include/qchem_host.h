@@ -47,6 +47,7 @@ typedef struct qcscf_result {
     int32_t converged;
     int32_t nbf;
     double t_total, t_jk, t_diag, t_setup;
+    int64_t quartets, quartets_fp32;  /* shell quartets over all Fock builds; of those in FP32 */
 } qcscf_result;
 
 /* Folded coefficients and ns for the shells (the conventions the engine expects). */

@@ -82,7 +82,8 @@ class _ScfResult(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("e_nuc", ctypes.c_double), ("e_one", ctypes.c_double),
                 ("e_coul", ctypes.c_double), ("e_exch", ctypes.c_double), ("iterations", ctypes.c_int32),
                 ("converged", ctypes.c_int32), ("nbf", ctypes.c_int32), ("t_total", ctypes.c_double),
-                ("t_jk", ctypes.c_double), ("t_diag", ctypes.c_double), ("t_setup", ctypes.c_double)]
+                ("t_jk", ctypes.c_double), ("t_diag", ctypes.c_double), ("t_setup", ctypes.c_double),
+                ("quartets", ctypes.c_int64), ("quartets_fp32", ctypes.c_int64)]
 
 
 QCJK_SYMBOLS = ["qcjk_create", "qcjk_nbf", "qcjk_build", "qcjk_build_device", "qcjk_one_electron",
@@ -311,4 +312,5 @@ def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, f
     return {"energy": res.energy, "e_nuc": res.e_nuc, "e_one": res.e_one, "e_coul": res.e_coul,
             "e_exch": res.e_exch, "iterations": it, "converged": bool(res.converged), "D": Dout,
             "energy_history": eh[:it].copy(), "time_history": th[:it].copy(), "t_total": res.t_total,
-            "t_jk": res.t_jk, "t_diag": res.t_diag, "t_setup": res.t_setup}
+            "t_jk": res.t_jk, "t_diag": res.t_diag, "t_setup": res.t_setup, "quartets": res.quartets,
+            "fp32_fraction": res.quartets_fp32 / max(1, res.quartets)}

@@ -27,7 +27,10 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-value",
     "-I" + INCLUDE, "-I" + CSRC,
 ]
-HIP_SOURCES = ["qcjk_eri.hip", "qcjk_setup.hip", "qcjk_api.hip"]
+# longest compiles first (they bound the parallel build)
+HIP_SOURCES = ["qcjk_k_light_c.hip", "qcjk_k_light_b.hip", "qcjk_k_lightc.hip", "qcjk_k_light_a.hip",
+               "qcjk_k_heavy2_c.hip", "qcjk_k_heavy2_d.hip", "qcjk_k_heavy2_b.hip", "qcjk_k_heavy2_a.hip",
+               "qcjk_k_heavy.hip", "qcjk_eri.hip", "qcjk_setup.hip", "qcjk_api.hip"]
 HOST_SOURCES = ["linalg.cpp", "scf.cpp", "capi.cpp"]
 
 
@@ -45,7 +48,7 @@ def build(verbose=False):
         obj = os.path.join(OBJ, src.replace(".hip", ".o"))
         jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
     hip_objs = [j[-1] for j in jobs]
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         for out in ex.map(_run, jobs):
             if verbose and out.strip():
                 print(out)

@@ -141,6 +141,20 @@ QC_HD void boys_f32(float T, const double* __restrict__ tab, float* F)
     }
 }
 
+// Mixed precision: F[0..L] as float.  T >= 40 keeps the FP64 asymptote (large-argument Boys stays
+// FP64); below, an FP32 Taylor step and FP32 downward recursion.
+template <int L>
+QC_HD void boys_mixed(double T, const double* __restrict__ tab, float* F)
+{
+    if (T >= kBoysTMax) {
+        double Fd[L + 1];
+        boys<L>(T, tab, Fd);
+        static_for<0, L + 1>([&](auto j) { F[decltype(j)::value] = (float)Fd[decltype(j)::value]; });
+    } else {
+        boys_f32<L>((float)T, tab, F);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Hermite expansion coefficients, one axis: E[i][j][t], i<=LA, j<=LB, t<=i+j (t=i+j+1 kept 0).
 // E^{i+1,j}_t = E^{ij}_{t-1}/(2p) + PA E^{ij}_t + (t+1) E^{ij}_{t+1}, and the same with PB for j.

@@ -27,51 +27,61 @@ struct H2Cfg {
     static constexpr int IB = XI >= 2304 ? 1 : (2304 / XI > 9 ? 9 : 2304 / XI);  // bra prim pairs per X batch
 };
 
-template <int LA, int LB, int LC, int LD>
+// Per-wavefront LDS of the R batch and the ket-transformed block, in the quartet's arithmetic
+// type.  The R recursion table and the contracted block I (always FP64) live outside it.
+template <int LA, int LB, int LC, int LD, typename RT>
 struct H2Smem {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD>;
-    double R[2][C::KB][S::NR];
-    double top[C::KB][S::L + 1];  // (-2 alpha)^j F_j per batch slot
-    double geo[C::KB][4];         // P-Q (x,y,z), lambda per batch slot
-    double X[C::IB][S::NCD][S::NHAB];
-    double I[S::NI];
-    int rtab[S::NR];              // R recursion decode per entry: dir | i1<<2 | i2<<10 | c2<<18
+    RT R[2][C::KB][S::NR];
+    RT top[C::KB][S::L + 1];  // (-2 alpha)^j F_j per batch slot
+    RT geo[C::KB][4];         // P-Q (x,y,z), lambda per batch slot
+    RT X[C::IB][S::NCD][S::NHAB];
 };
 
-// Build the per-entry R recursion table (once per kernel).
+// FP64 and FP32 quartets of one kernel share the LDS.
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void heavy2_init(H2Smem<LA, LB, LC, LD>& sm, int lane)
+union H2Union {
+    H2Smem<LA, LB, LC, LD, double> d;
+    H2Smem<LA, LB, LC, LD, float> f;
+};
+
+// Build the per-entry R recursion table (once per kernel): dir | i1<<2 | i2<<10 | c2<<18.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void heavy2_init(int* rtab, int lane)
 {
     using S = HeavyShape<LA, LB, LC, LD>;
     for (int e = lane; e < S::NR; e += 64) {
         int t, u, v, dir = 0, i1 = 0, i2 = 0, c2 = 0;
         hdec(e, t, u, v);
-        if (e == 0) { sm.rtab[e] = 0; continue; }
+        if (e == 0) { rtab[e] = 0; continue; }
         if (t > 0) { dir = 0; i1 = hidx(t - 1, u, v); if (t > 1) { i2 = hidx(t - 2, u, v); c2 = t - 1; } }
         else if (u > 0) { dir = 1; i1 = hidx(0, u - 1, v); if (u > 1) { i2 = hidx(0, u - 2, v); c2 = u - 1; } }
         else { dir = 2; i1 = hidx(0, 0, v - 1); if (v > 1) { i2 = hidx(0, 0, v - 2); c2 = v - 1; } }
-        sm.rtab[e] = dir | (i1 << 2) | (i2 << 10) | (c2 << 18);
+        rtab[e] = dir | (i1 << 2) | (i2 << 10) | (c2 << 18);
     }
     __syncthreads();
 }
 
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD>& sm, int lane, int t0,
-                                               int u0, int v0, int4 pi, double4 pg, int4 qi, double4 qg)
+// One contracted quartet into sI (FP64, NAB x NCD).  RT = float evaluates the primitive integrals
+// in FP32 (the Boys function at T >= 40 and the (-2 alpha)^j F_j and lambda products in FP64).
+template <int LA, int LB, int LC, int LD, typename RT>
+__device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD, RT>& sm,
+                                               const int* __restrict__ rtab, double* sI, int lane, int t0, int u0,
+                                               int v0, int4 pi, double4 pg, int4 qi, double4 qg)
 {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD>;
     constexpr int NHCD = nherm(S::LCD);
-    for (int e = lane; e < S::NI; e += 64) sm.I[e] = 0.0;
+    for (int e = lane; e < S::NI; e += 64) sI[e] = 0.0;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;
     const bool xl = lane < S::NHAB;
     for (int ij0 = 0; ij0 < npab; ij0 += C::IB) {
         const int nib = npab - ij0 < C::IB ? npab - ij0 : C::IB;
         for (int ii = 0; ii < nib; ++ii) {
             const double4 b0 = A.pp0[pi.z + ij0 + ii], b1 = A.pp1[pi.z + ij0 + ii];
-            double xacc[S::NCD];
-            static_for<0, S::NCD>([&](auto k) { xacc[decltype(k)::value] = 0.0; });
+            RT xacc[S::NCD];
+            static_for<0, S::NCD>([&](auto k) { xacc[decltype(k)::value] = RT(0); });
             for (int kl0 = 0; kl0 < npcd; kl0 += C::KB) {
                 const int nb = npcd - kl0 < C::KB ? npcd - kl0 : C::KB;
                 __syncthreads();  // readers of the previous batch (R, top, geo) are done
@@ -79,18 +89,20 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     const double4 k0 = A.pp0[qi.z + kl0 + lane], k1 = A.pp1[qi.z + kl0 + lane];
                     const double p = b0.w, q = k0.w, pq = p + q, alpha = p * q / pq;
                     const double X = b0.x - k0.x, Y = b0.y - k0.y, Z = b0.z - k0.z;
-                    double F[S::L + 1];
-                    boys<S::L>(alpha * (X * X + Y * Y + Z * Z), A.boys, F);
+                    const double T = alpha * (X * X + Y * Y + Z * Z);
+                    RT F[S::L + 1];
+                    if constexpr (std::is_same_v<RT, double>) boys<S::L>(T, A.boys, F);
+                    else boys_mixed<S::L>(T, A.boys, F);
                     double pw = 1.0;
                     static_for<0, S::L + 1>([&](auto jj) {
                         constexpr int j = decltype(jj)::value;
-                        sm.top[lane][j] = pw * F[j];
+                        sm.top[lane][j] = RT(pw * (double)F[j]);
                         pw *= -2.0 * alpha;
                     });
-                    sm.geo[lane][0] = X;
-                    sm.geo[lane][1] = Y;
-                    sm.geo[lane][2] = Z;
-                    sm.geo[lane][3] = kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w;
+                    sm.geo[lane][0] = RT(X);
+                    sm.geo[lane][1] = RT(Y);
+                    sm.geo[lane][2] = RT(Z);
+                    sm.geo[lane][3] = RT(kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w);
                 }
                 __syncthreads();
                 static_for<0, S::L + 1>([&](auto jj) {
@@ -98,14 +110,14 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     constexpr int nv = nherm(S::L - j);
                     for (int idx = lane; idx < nb * nv; idx += 64) {
                         const int s = idx / nv, e = idx - s * nv;
-                        double* cur = sm.R[j & 1][s];
+                        RT* cur = sm.R[j & 1][s];
                         if (e == 0) {
                             cur[0] = sm.top[s][j];
                         } else {
-                            const double* prev = sm.R[(j + 1) & 1][s];
-                            const int d = sm.rtab[e];
+                            const RT* prev = sm.R[(j + 1) & 1][s];
+                            const int d = rtab[e];
                             const int dir = d & 3, i1 = (d >> 2) & 255, i2 = (d >> 10) & 255, c2 = (d >> 18) & 15;
-                            cur[e] = sm.geo[s][dir] * prev[i1] + (double)c2 * prev[i2];
+                            cur[e] = sm.geo[s][dir] * prev[i1] + (RT)c2 * prev[i2];
                         }
                     }
                     __syncthreads();
@@ -113,13 +125,13 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                 if (xl) {
                     for (int s = 0; s < nb; ++s) {
                         const double4 k0 = A.pp0[qi.z + kl0 + s], k1 = A.pp1[qi.z + kl0 + s];
-                        HermiteAxis<LC, LD, double> ecd[3];
-                        const double c12 = 0.5 / k0.w;
-                        ecd[0].build(c12, k1.x, k1.x + qg.x);
-                        ecd[1].build(c12, k1.y, k1.y + qg.y);
-                        ecd[2].build(c12, k1.z, k1.z + qg.z);
-                        const double* R0 = sm.R[0][s];
-                        double rw[NHCD];
+                        HermiteAxis<LC, LD, RT> ecd[3];
+                        const RT c12 = RT(0.5 / k0.w);
+                        ecd[0].build(c12, RT(k1.x), RT(k1.x + qg.x));
+                        ecd[1].build(c12, RT(k1.y), RT(k1.y + qg.y));
+                        ecd[2].build(c12, RT(k1.z), RT(k1.z + qg.z));
+                        const RT* R0 = sm.R[0][s];
+                        RT rw[NHCD];
                         static_for<0, S::LCD + 1>([&](auto ta) {
                             constexpr int tau = decltype(ta)::value;
                             static_for<0, S::LCD + 1 - tau>([&](auto na) {
@@ -130,22 +142,22 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                                 });
                             });
                         });
-                        const double lam = sm.geo[s][3];
+                        const RT lam = sm.geo[s][3];
                         static_for<0, S::NC>([&](auto cc) {
                             constexpr int c = decltype(cc)::value;
                             static_for<0, S::ND>([&](auto dd) {
                                 constexpr int d = decltype(dd)::value;
                                 constexpr int cx = comp(LC, c, 0), cy = comp(LC, c, 1), cz = comp(LC, c, 2);
                                 constexpr int dx = comp(LD, d, 0), dy = comp(LD, d, 1), dz = comp(LD, d, 2);
-                                double acc = 0.0;
+                                RT acc = RT(0);
                                 static_for<0, cx + dx + 1>([&](auto t2) {
                                     constexpr int tau = decltype(t2)::value;
                                     static_for<0, cy + dy + 1>([&](auto u2) {
                                         constexpr int nu = decltype(u2)::value;
-                                        const double exy = ecd[0].e[cx][dx][tau] * ecd[1].e[cy][dy][nu];
+                                        const RT exy = ecd[0].e[cx][dx][tau] * ecd[1].e[cy][dy][nu];
                                         static_for<0, cz + dz + 1>([&](auto v2) {
                                             constexpr int phi = decltype(v2)::value;
-                                            const double term = exy * ecd[2].e[cz][dz][phi] * rw[hidx(tau, nu, phi)];
+                                            const RT term = exy * ecd[2].e[cz][dz][phi] * rw[hidx(tau, nu, phi)];
                                             if constexpr (((tau + nu + phi) & 1) != 0) acc -= term;
                                             else acc += term;
                                         });
@@ -163,12 +175,12 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
         for (int pr = lane; pr < nib * S::NCD; pr += 64) {
             const int ii = pr / S::NCD, cd = pr - ii * S::NCD;
             const double4 b0 = A.pp0[pi.z + ij0 + ii], b1 = A.pp1[pi.z + ij0 + ii];
-            HermiteAxis<LA, LB, double> eab[3];
-            const double a12 = 0.5 / b0.w;
-            eab[0].build(a12, b1.x, b1.x + pg.x);
-            eab[1].build(a12, b1.y, b1.y + pg.y);
-            eab[2].build(a12, b1.z, b1.z + pg.z);
-            double xw[S::NHAB];
+            HermiteAxis<LA, LB, RT> eab[3];
+            const RT a12 = RT(0.5 / b0.w);
+            eab[0].build(a12, RT(b1.x), RT(b1.x + pg.x));
+            eab[1].build(a12, RT(b1.y), RT(b1.y + pg.y));
+            eab[2].build(a12, RT(b1.z), RT(b1.z + pg.z));
+            RT xw[S::NHAB];
             static_for<0, S::NHAB>([&](auto h) { xw[decltype(h)::value] = sm.X[ii][cd][decltype(h)::value]; });
             static_for<0, S::NA>([&](auto aa) {
                 constexpr int a = decltype(aa)::value;
@@ -176,19 +188,19 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                 static_for<0, S::NB>([&](auto bb) {
                     constexpr int b = decltype(bb)::value;
                     constexpr int bx = comp(LB, b, 0), by = comp(LB, b, 1), bz = comp(LB, b, 2);
-                    double acc = 0.0;
+                    RT acc = RT(0);
                     static_for<0, ax + bx + 1>([&](auto tt) {
                         constexpr int t = decltype(tt)::value;
                         static_for<0, ay + by + 1>([&](auto uu) {
                             constexpr int u = decltype(uu)::value;
-                            const double exy = eab[0].e[ax][bx][t] * eab[1].e[ay][by][u];
+                            const RT exy = eab[0].e[ax][bx][t] * eab[1].e[ay][by][u];
                             static_for<0, az + bz + 1>([&](auto vv) {
                                 constexpr int v = decltype(vv)::value;
                                 acc += exy * eab[2].e[az][bz][v] * xw[hidx(t, u, v)];
                             });
                         });
                     });
-                    atomicAdd(&sm.I[(a * S::NB + b) * S::NCD + cd], acc);
+                    atomicAdd(&sI[(a * S::NB + b) * S::NCD + cd], (double)acc);
                 });
             });
         }

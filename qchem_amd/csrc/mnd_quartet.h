@@ -22,20 +22,20 @@ constexpr bool kLightTwoPhase = true;
 // With eab_mem (a wavefront-private LDS slot) the bra Hermite tables -- identical in every lane,
 // since a wavefront shares its bra pair -- are kept there instead of in registers: every lane
 // stores the same values and reads back its own stores, so no cross-lane ordering is needed.
-template <int LA, int LB, int LC, int LD, bool BRA_MEM = false>
-QC_HD void light_compute(const BuildArgs& A, int4 pi, double4 pg, int4 qi, double4 qg, double* I,
-                         HermiteAxis<LA, LB, double>* eab_mem = nullptr)
+template <int LA, int LB, int LC, int LD, bool BRA_MEM = false, typename RT = double>
+QC_HD void light_compute(const BuildArgs& A, int4 pi, double4 pg, int4 qi, double4 qg, RT* I,
+                         HermiteAxis<LA, LB, RT>* eab_mem = nullptr)
 {
     constexpr int L = LA + LB + LC + LD;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;
     for (int ij = 0; ij < npab; ++ij) {
         const double4 b0 = A.pp0[pi.z + ij], b1 = A.pp1[pi.z + ij];
-        HermiteAxis<LA, LB, double> eloc[3];
-        const double a12 = 0.5 / b0.w;
-        eloc[0].build(a12, b1.x, b1.x + pg.x);
-        eloc[1].build(a12, b1.y, b1.y + pg.y);
-        eloc[2].build(a12, b1.z, b1.z + pg.z);
-        const HermiteAxis<LA, LB, double>* eab = eloc;
+        HermiteAxis<LA, LB, RT> eloc[3];
+        const RT a12 = RT(0.5 / b0.w);
+        eloc[0].build(a12, RT(b1.x), RT(b1.x + pg.x));
+        eloc[1].build(a12, RT(b1.y), RT(b1.y + pg.y));
+        eloc[2].build(a12, RT(b1.z), RT(b1.z + pg.z));
+        const HermiteAxis<LA, LB, RT>* eab = eloc;
         if constexpr (BRA_MEM) {
             eloc[0].store(eab_mem[0]);
             eloc[1].store(eab_mem[1]);
@@ -44,42 +44,43 @@ QC_HD void light_compute(const BuildArgs& A, int4 pi, double4 pg, int4 qi, doubl
         }
         for (int kl = 0; kl < npcd; ++kl) {
             const double4 k0 = A.pp0[qi.z + kl], k1 = A.pp1[qi.z + kl];
-            HermiteAxis<LC, LD, double> ecd[3];
-            const double c12 = 0.5 / k0.w;
-            ecd[0].build(c12, k1.x, k1.x + qg.x);
-            ecd[1].build(c12, k1.y, k1.y + qg.y);
-            ecd[2].build(c12, k1.z, k1.z + qg.z);
+            HermiteAxis<LC, LD, RT> ecd[3];
+            const RT c12 = RT(0.5 / k0.w);
+            ecd[0].build(c12, RT(k1.x), RT(k1.x + qg.x));
+            ecd[1].build(c12, RT(k1.y), RT(k1.y + qg.y));
+            ecd[2].build(c12, RT(k1.z), RT(k1.z + qg.z));
             const double p = b0.w, q = k0.w, pq = p + q, alpha = p * q / pq;
             const double X = b0.x - k0.x, Y = b0.y - k0.y, Z = b0.z - k0.z;
             const double T = alpha * (X * X + Y * Y + Z * Z);
-            double F[L + 1];
-            boys<L>(T, A.boys, F);
-            double R[nherm(L)];
-            rtuv<L, double>(alpha, X, Y, Z, F, R);
-            const double lam = kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w;
+            RT F[L + 1];
+            if constexpr (std::is_same_v<RT, double>) boys<L>(T, A.boys, F);
+            else boys_mixed<L>(T, A.boys, F);
+            RT R[nherm(L)];
+            rtuv<L, RT>(RT(alpha), RT(X), RT(Y), RT(Z), F, R);
+            const RT lam = RT(kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w);
             if constexpr (kLightTwoPhase<LA, LB, LC, LD>)
-                prim_quartet_two_phase<LA, LB, LC, LD, double>(eab, ecd, R, lam, I);
+                prim_quartet_two_phase<LA, LB, LC, LD, RT>(eab, ecd, R, lam, I);
             else
-                prim_quartet<LA, LB, LC, LD, double>(eab, ecd, R, lam, I);
+                prim_quartet<LA, LB, LC, LD, RT>(eab, ecd, R, lam, I);
         }
     }
 }
 
 // Lane-per-quartet, one ket first-component chunk CI: I[(a*NB+b)*ND+d] for c = CI, all primitives.
-template <int LA, int LB, int LC, int LD, int CI, bool BRA_MEM>
-QC_HD void light_compute_chunk(const BuildArgs& A, int4 pi, double4 pg, int4 qi, double4 qg, double* I,
-                               HermiteAxis<LA, LB, double>* eab_mem)
+template <int LA, int LB, int LC, int LD, int CI, bool BRA_MEM, typename RT = double>
+QC_HD void light_compute_chunk(const BuildArgs& A, int4 pi, double4 pg, int4 qi, double4 qg, RT* I,
+                               HermiteAxis<LA, LB, RT>* eab_mem)
 {
     constexpr int L = LA + LB + LC + LD;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;
     for (int ij = 0; ij < npab; ++ij) {
         const double4 b0 = A.pp0[pi.z + ij], b1 = A.pp1[pi.z + ij];
-        HermiteAxis<LA, LB, double> eloc[3];
-        const double a12 = 0.5 / b0.w;
-        eloc[0].build(a12, b1.x, b1.x + pg.x);
-        eloc[1].build(a12, b1.y, b1.y + pg.y);
-        eloc[2].build(a12, b1.z, b1.z + pg.z);
-        const HermiteAxis<LA, LB, double>* eab = eloc;
+        HermiteAxis<LA, LB, RT> eloc[3];
+        const RT a12 = RT(0.5 / b0.w);
+        eloc[0].build(a12, RT(b1.x), RT(b1.x + pg.x));
+        eloc[1].build(a12, RT(b1.y), RT(b1.y + pg.y));
+        eloc[2].build(a12, RT(b1.z), RT(b1.z + pg.z));
+        const HermiteAxis<LA, LB, RT>* eab = eloc;
         if constexpr (BRA_MEM) {
             eloc[0].store(eab_mem[0]);
             eloc[1].store(eab_mem[1]);
@@ -88,19 +89,21 @@ QC_HD void light_compute_chunk(const BuildArgs& A, int4 pi, double4 pg, int4 qi,
         }
         for (int kl = 0; kl < npcd; ++kl) {
             const double4 k0 = A.pp0[qi.z + kl], k1 = A.pp1[qi.z + kl];
-            HermiteAxis<LC, LD, double> ecd[3];
-            const double c12 = 0.5 / k0.w;
-            ecd[0].build(c12, k1.x, k1.x + qg.x);
-            ecd[1].build(c12, k1.y, k1.y + qg.y);
-            ecd[2].build(c12, k1.z, k1.z + qg.z);
+            HermiteAxis<LC, LD, RT> ecd[3];
+            const RT c12 = RT(0.5 / k0.w);
+            ecd[0].build(c12, RT(k1.x), RT(k1.x + qg.x));
+            ecd[1].build(c12, RT(k1.y), RT(k1.y + qg.y));
+            ecd[2].build(c12, RT(k1.z), RT(k1.z + qg.z));
             const double p = b0.w, q = k0.w, pq = p + q, alpha = p * q / pq;
             const double X = b0.x - k0.x, Y = b0.y - k0.y, Z = b0.z - k0.z;
-            double F[L + 1];
-            boys<L>(alpha * (X * X + Y * Y + Z * Z), A.boys, F);
-            double R[nherm(L)];
-            rtuv<L, double>(alpha, X, Y, Z, F, R);
-            const double lam = kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w;
-            prim_quartet_chunk<LA, LB, LC, LD, CI, double>(eab, ecd, R, lam, I);
+            const double T = alpha * (X * X + Y * Y + Z * Z);
+            RT F[L + 1];
+            if constexpr (std::is_same_v<RT, double>) boys<L>(T, A.boys, F);
+            else boys_mixed<L>(T, A.boys, F);
+            RT R[nherm(L)];
+            rtuv<L, RT>(RT(alpha), RT(X), RT(Y), RT(Z), F, R);
+            const RT lam = RT(kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w);
+            prim_quartet_chunk<LA, LB, LC, LD, CI, RT>(eab, ecd, R, lam, I);
         }
     }
 }

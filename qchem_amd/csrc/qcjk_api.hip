@@ -25,6 +25,7 @@ using namespace qcjk;
 namespace {
 
 constexpr double kPruneTol = 1e-30;  // create-time: drop quartets whose Schwarz product is below this
+constexpr double kFp32MaxLog10 = 30.0;  // FP32 quartets only where (2 alpha)^L < 1e30 (FP32 max 3.4e38)
 
 // Variants: 0 = lane per quartet (64 ket pairs per unit), 1 = wavefront per quartet with LDS
 // transforms, 2 = wavefront per quartet with register transforms (1 ket pair per unit),
@@ -42,6 +43,7 @@ struct ClassPairWork {
     int* d_nsurv = nullptr;
     double prim_flops = 0;     // per primitive quartet
     double digest_flops = 0;   // per contracted quartet
+    bool fp32_ok = true;       // (2 alpha_max)^L of the class fits FP32 with headroom
     bool has(int v) const
     {
         return v == 1 || v == 2 || (v == 0 && info->light != nullptr) || (v == 3 && info->lightc != nullptr);
@@ -360,6 +362,18 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     HIPCHK(hipMemcpy(h->d_pinfo, pinfo.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_pgeo, pgeo.data(), npair * sizeof(double4), hipMemcpyHostToDevice));
 
+    // largest primitive-pair exponent p = a_i + b_j per pair class (FP32 range guard)
+    double class_pmax[kNumPairClasses] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < npair; ++k) {
+        double amax[2] = {0.0, 0.0};
+        const int sh[2] = {h->pair_first[k], h->pair_second[k]};
+        for (int x = 0; x < 2; ++x)
+            for (int i = 0; i < h->snp[sh[x]]; ++i) amax[x] = std::max(amax[x], h->exps[h->spoff[sh[x]] + i]);
+        int c = 0;
+        while (k >= h->coff[c + 1]) ++c;
+        class_pmax[c] = std::max(class_pmax[c], amax[0] + amax[1]);
+    }
+
     // work lists per (bra class >= ket class)
     for (int bc = 0; bc < kNumPairClasses; ++bc)
         for (int kc = 0; kc <= bc; ++kc) {
@@ -371,6 +385,12 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
             w.info = class_kernel(bc, kc);
             if (!w.info) return QCJK_ERR_UNSUPPORTED;
             w.variant = force_heavy ? 1 : ((w.info->tile == 64 && w.info->light) ? 0 : 2);
+            {
+                // FP32 range guard: R^{(0)}_{tuv} carries up to (2 alpha)^L with alpha <= min(p, q).
+                const int L = w.info->la + w.info->lb + w.info->lc + w.info->ld;
+                const double amax = std::min(class_pmax[bc], class_pmax[kc]);
+                w.fp32_ok = L * std::log10(std::max(2.0 * amax, 1.0)) < kFp32MaxLog10;
+            }
             w.prim_flops = prim_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             w.digest_flops = digest_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             std::vector<long long> start0(nb + 1, 0), start1(nb + 1, 0);
@@ -532,7 +552,6 @@ int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, 
     if (!h || !dD) return QCJK_ERR_ARG;
     if (!dJ) flags &= ~(unsigned)QCJK_J;
     if (!dK) flags &= ~(unsigned)QCJK_K;
-    if (flags & QCJK_MIXED) return QCJK_ERR_UNSUPPORTED;
     HIPCHK(hipSetDevice(h->device));
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     const int n = h->n;
@@ -554,6 +573,7 @@ int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, 
             const ClassPairWork& w = h->work[i];
             const ClassPairArgs c = class_args(h, w, w.variant);
             a.counters = h->d_counters + 3 * i;
+            a.flags = w.fp32_ok ? flags : (flags & ~(unsigned)QCJK_MIXED);
             if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i], s));
             HIPCHK(w.launcher(w.variant)(a, c, s));
             if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i + 1], s));

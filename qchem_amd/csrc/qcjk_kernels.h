@@ -1,0 +1,476 @@
+// qcjk_kernels.h -- J/K class kernels for gfx950 and their launchers (templates).  Included only by
+// the kernel translation units (qcjk_k_*.hip), each of which instantiates one kernel shape for its
+// classes, so the heavy templates compile in parallel; the dispatch table (qcjk_eri.hip) sees
+// only the declarations in qcjk_launch.h.
+//
+// Replaces, for the molecular Gaussian path, the reference's per-element ERI loop and ERI4
+// contraction (Molecule/IrrepBasisSet.C:138-183, Internal/Imp/ERI4.C:10-42): every unique shell
+// quartet (8-fold symmetry) is evaluated once and digested straight into J and K, with FP64
+// hardware atomics into the unsymmetrised accumulators Jt, Kt:
+//   s = (A==B ? 1/2 : 1)(C==D ? 1/2 : 1)(AB==CD ? 1/2 : 1)
+//   Jt_ab += 2s sum_cd I D_cd   Jt_cd += 2s sum_ab I D_ab
+//   Kt_ac += s sum_bd I D_bd    Kt_bc += s sum_ad I D_ad   Kt_ad += s sum_bc I D_bc   Kt_bd += s sum_ac I D_ac
+// and J = Jt + Jt^T, K = Kt + Kt^T afterwards (qcjk_api.hip).
+//
+// Kernel shapes (the variant numbers of qcjk_set_variant):
+//   0 k_jk_light   lane per contracted quartet.  A wavefront owns one bra pair and 64 consecutive
+//                  ket pairs, so the J_ab contribution is reduced across the wavefront before one
+//                  atomic per component; everything else is register resident.
+//   1 k_jk_heavy   wavefront per quartet, LDS-staged transforms (mnd_heavy.h).
+//   2 k_jk_heavy2  wavefront per quartet, register-resident transforms, batched R (mnd_heavy2.h).
+//   3 k_jk_lightc  lane per quartet, chunked over the ket's first component.
+// With QCJK_MIXED, variants 0, 2 and 3 evaluate small quartets' primitive integrals in FP32
+// (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.
+#pragma once
+
+#include "qcjk_launch.h"
+#include "mnd_quartet.h"
+#include "mnd_heavy2.h"
+
+namespace qcjk {
+
+__device__ __forceinline__ int find_unit_owner(const long long* __restrict__ start, int n, long long u)
+{
+    int lo = 0, hi = n;  // start[lo] <= u < start[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (start[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Mixed precision, lane-per-quartet kernels: the wavefront runs its primitive integrals in FP32
+// when every active lane's quartet has Q_ab Q_cd max|D'| below mixed_tol (one ballot, so the
+// choice is uniform and the two paths never diverge within a wavefront).  Ket pairs are sorted by
+// descending bound, so the choice is almost always the per-lane one.
+__device__ __forceinline__ bool wave_fp32(const BuildArgs& A, bool active, double4 pg, double4 qg)
+{
+    if (!(A.flags & kFlagMixed)) return false;
+    const bool big = active && !(pg.w * qg.w * (*A.dmax) < A.mixed_tol);
+    return __ballot(big) == 0;
+}
+
+__device__ __forceinline__ void count_lane_quartets(const BuildArgs& A, bool active, bool fp32, int lane, int4 pi,
+                                                    int4 qi)
+{
+    const unsigned long long m = __ballot(active);
+    if (lane == 0 && m) {
+        const int cnt = __popcll(m);
+        atomicAdd(&A.counters[0], (unsigned long long)cnt);
+        if (fp32) atomicAdd(&A.counters[1], (unsigned long long)cnt);
+        atomicAdd(&A.counters[2], (unsigned long long)cnt * (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lane-per-quartet kernel
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, bool BRA_MEM>
+__global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB, NCD = NC * ND;
+    __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
+    __shared__ HermiteAxis<LA, LB, float> sEab32[BRA_MEM ? 4 : 1][3];
+    const int lane = threadIdx.x & 63;
+    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (unit >= C.nunits) return;  // uniform per wavefront
+    const int P = find_unit_owner(C.unit_start, C.nbra, unit);
+    const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
+    bool active = q < C.nsurv[P];
+    const int Q = active ? q : 0;
+    const int4 pi = A.pinfo[C.bra_off + P];
+    const double4 pg = A.pgeo[C.bra_off + P];
+    const int4 qi = A.pinfo[C.ket_off + Q];
+    const double4 qg = A.pgeo[C.ket_off + Q];
+    if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
+    const bool fp32 = wave_fp32(A, active, pg, qg);
+    const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
+
+    double I[NAB * NCD];
+    static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = 0.0; });
+    if (active) {
+        if (fp32) {
+            float If[NAB * NCD];
+            static_for<0, NAB * NCD>([&](auto i) { If[decltype(i)::value] = 0.0f; });
+            light_compute<LA, LB, LC, LD, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
+            static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
+        } else {
+            light_compute<LA, LB, LC, LD, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
+        }
+    }
+
+    double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+    if (!active) s = 0.0;
+    const int n = A.n;
+    const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
+    const double* __restrict__ D = A.D;
+
+    if (A.flags & kFlagJ) {
+        double dcd[NCD];
+        static_for<0, NCD>([&](auto i) {
+            constexpr int c = decltype(i)::value / ND, d = decltype(i)::value % ND;
+            dcd[decltype(i)::value] = active ? D[(oc + c) * n + od + d] : 0.0;
+        });
+        static_for<0, NAB>([&](auto ab) {
+            constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+            double v = 0.0;
+            static_for<0, NCD>([&](auto cd) { v += I[decltype(ab)::value * NCD + decltype(cd)::value] * dcd[decltype(cd)::value]; });
+            v = wave_sum(2.0 * s * v);
+            if (lane == 0) unsafeAtomicAdd(&A.Jt[(oa + a) * n + ob + b], v);
+        });
+        double dab[NAB];
+        static_for<0, NAB>([&](auto i) {
+            constexpr int a = decltype(i)::value / NB, b = decltype(i)::value % NB;
+            dab[decltype(i)::value] = D[(oa + a) * n + ob + b];
+        });
+        if (active) {
+            static_for<0, NCD>([&](auto cd) {
+                constexpr int c = decltype(cd)::value / ND, d = decltype(cd)::value % ND;
+                double v = 0.0;
+                static_for<0, NAB>([&](auto ab) { v += I[decltype(ab)::value * NCD + decltype(cd)::value] * dab[decltype(ab)::value]; });
+                unsafeAtomicAdd(&A.Jt[(oc + c) * n + od + d], 2.0 * s * v);
+            });
+        }
+    }
+    if ((A.flags & kFlagK) && active) {
+        // Kt_ac += s sum_bd I D_bd
+        {
+            double dd[NB * ND];
+            static_for<0, NB * ND>([&](auto i) {
+                constexpr int b = decltype(i)::value / ND, d = decltype(i)::value % ND;
+                dd[decltype(i)::value] = D[(ob + b) * n + od + d];
+            });
+            static_for<0, NA * NC>([&](auto x) {
+                constexpr int a = decltype(x)::value / NC, c = decltype(x)::value % NC;
+                double v = 0.0;
+                static_for<0, NB * ND>([&](auto y) {
+                    constexpr int b = decltype(y)::value / ND, d = decltype(y)::value % ND;
+                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                });
+                unsafeAtomicAdd(&A.Kt[(oa + a) * n + oc + c], s * v);
+            });
+        }
+        // Kt_bc += s sum_ad I D_ad
+        {
+            double dd[NA * ND];
+            static_for<0, NA * ND>([&](auto i) {
+                constexpr int a = decltype(i)::value / ND, d = decltype(i)::value % ND;
+                dd[decltype(i)::value] = D[(oa + a) * n + od + d];
+            });
+            static_for<0, NB * NC>([&](auto x) {
+                constexpr int b = decltype(x)::value / NC, c = decltype(x)::value % NC;
+                double v = 0.0;
+                static_for<0, NA * ND>([&](auto y) {
+                    constexpr int a = decltype(y)::value / ND, d = decltype(y)::value % ND;
+                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                });
+                unsafeAtomicAdd(&A.Kt[(ob + b) * n + oc + c], s * v);
+            });
+        }
+        // Kt_ad += s sum_bc I D_bc
+        {
+            double dd[NB * NC];
+            static_for<0, NB * NC>([&](auto i) {
+                constexpr int b = decltype(i)::value / NC, c = decltype(i)::value % NC;
+                dd[decltype(i)::value] = D[(ob + b) * n + oc + c];
+            });
+            static_for<0, NA * ND>([&](auto x) {
+                constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
+                double v = 0.0;
+                static_for<0, NB * NC>([&](auto y) {
+                    constexpr int b = decltype(y)::value / NC, c = decltype(y)::value % NC;
+                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                });
+                unsafeAtomicAdd(&A.Kt[(oa + a) * n + od + d], s * v);
+            });
+        }
+        // Kt_bd += s sum_ac I D_ac
+        {
+            double dd[NA * NC];
+            static_for<0, NA * NC>([&](auto i) {
+                constexpr int a = decltype(i)::value / NC, c = decltype(i)::value % NC;
+                dd[decltype(i)::value] = D[(oa + a) * n + oc + c];
+            });
+            static_for<0, NB * ND>([&](auto x) {
+                constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
+                double v = 0.0;
+                static_for<0, NA * NC>([&](auto y) {
+                    constexpr int a = decltype(y)::value / NC, c = decltype(y)::value % NC;
+                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                });
+                unsafeAtomicAdd(&A.Kt[(ob + b) * n + od + d], s * v);
+            });
+        }
+    }
+    if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lane-per-quartet kernel, chunked over the ket's first component: the primitive loop runs once
+// per chunk (R recomputed), so only NAB*ND accumulators are live.  J_ab, K_ad, K_bd accumulate
+// across chunks; J_cd, K_ac, K_bc of a chunk are final when it ends.
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD, bool BRA_MEM>
+__global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB;
+    __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
+    __shared__ HermiteAxis<LA, LB, float> sEab32[BRA_MEM ? 4 : 1][3];
+    const int lane = threadIdx.x & 63;
+    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (unit >= C.nunits) return;  // uniform per wavefront
+    const int P = find_unit_owner(C.unit_start, C.nbra, unit);
+    const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
+    bool active = q < C.nsurv[P];
+    const int Q = active ? q : 0;
+    const int4 pi = A.pinfo[C.bra_off + P];
+    const double4 pg = A.pgeo[C.bra_off + P];
+    const int4 qi = A.pinfo[C.ket_off + Q];
+    const double4 qg = A.pgeo[C.ket_off + Q];
+    if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
+    const bool fp32 = wave_fp32(A, active, pg, qg);
+    const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
+    double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+    if (!active) s = 0.0;
+    const int n = A.n;
+    const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
+    const double* __restrict__ D = A.D;
+    const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    double jab[NAB], kad[NA * ND], kbd[NB * ND];
+    static_for<0, NAB>([&](auto i) { jab[decltype(i)::value] = 0.0; });
+    static_for<0, NA * ND>([&](auto i) { kad[decltype(i)::value] = 0.0; });
+    static_for<0, NB * ND>([&](auto i) { kbd[decltype(i)::value] = 0.0; });
+    static_for<0, NC>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        double I[NAB * ND];
+        static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = 0.0; });
+        if (active) {
+            if (fp32) {
+                float If[NAB * ND];
+                static_for<0, NAB * ND>([&](auto i) { If[decltype(i)::value] = 0.0f; });
+                light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
+                static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
+            } else {
+                light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
+            }
+        }
+        if (active && doJ) {
+            static_for<0, ND>([&](auto dd) {
+                constexpr int d = decltype(dd)::value;
+                const double dcd = D[(oc + c) * n + od + d];
+                double v = 0.0;
+                static_for<0, NAB>([&](auto ab) {
+                    constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+                    jab[decltype(ab)::value] += I[decltype(ab)::value * ND + d] * dcd;
+                    v += I[decltype(ab)::value * ND + d] * D[(oa + a) * n + ob + b];
+                });
+                unsafeAtomicAdd(&A.Jt[(oc + c) * n + od + d], 2.0 * s * v);
+            });
+        }
+        if (active && doK) {
+            static_for<0, NA>([&](auto aa) {  // Kt_ac += s sum_bd I D_bd
+                constexpr int a = decltype(aa)::value;
+                double v = 0.0;
+                static_for<0, NB>([&](auto bb) {
+                    constexpr int b = decltype(bb)::value;
+                    static_for<0, ND>([&](auto dd) {
+                        constexpr int d = decltype(dd)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + od + d];
+                    });
+                });
+                unsafeAtomicAdd(&A.Kt[(oa + a) * n + oc + c], s * v);
+            });
+            static_for<0, NB>([&](auto bb) {  // Kt_bc += s sum_ad I D_ad
+                constexpr int b = decltype(bb)::value;
+                double v = 0.0;
+                static_for<0, NA>([&](auto aa) {
+                    constexpr int a = decltype(aa)::value;
+                    static_for<0, ND>([&](auto dd) {
+                        constexpr int d = decltype(dd)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + od + d];
+                    });
+                });
+                unsafeAtomicAdd(&A.Kt[(ob + b) * n + oc + c], s * v);
+            });
+            static_for<0, NA>([&](auto aa) {  // Kt_ad += s sum_bc I D_bc (this chunk's c)
+                constexpr int a = decltype(aa)::value;
+                static_for<0, ND>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    double v = 0.0;
+                    static_for<0, NB>([&](auto bb) {
+                        constexpr int b = decltype(bb)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + oc + c];
+                    });
+                    kad[a * ND + d] += v;
+                });
+            });
+            static_for<0, NB>([&](auto bb) {  // Kt_bd += s sum_ac I D_ac (this chunk's c)
+                constexpr int b = decltype(bb)::value;
+                static_for<0, ND>([&](auto dd) {
+                    constexpr int d = decltype(dd)::value;
+                    double v = 0.0;
+                    static_for<0, NA>([&](auto aa) {
+                        constexpr int a = decltype(aa)::value;
+                        v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + oc + c];
+                    });
+                    kbd[b * ND + d] += v;
+                });
+            });
+        }
+    });
+    if (doJ) {
+        static_for<0, NAB>([&](auto ab) {
+            constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+            const double v = wave_sum(2.0 * s * jab[decltype(ab)::value]);
+            if (lane == 0) unsafeAtomicAdd(&A.Jt[(oa + a) * n + ob + b], v);
+        });
+    }
+    if (doK && active) {
+        static_for<0, NA * ND>([&](auto x) {
+            constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
+            unsafeAtomicAdd(&A.Kt[(oa + a) * n + od + d], s * kad[decltype(x)::value]);
+        });
+        static_for<0, NB * ND>([&](auto x) {
+            constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
+            unsafeAtomicAdd(&A.Kt[(ob + b) * n + od + d], s * kbd[decltype(x)::value]);
+        });
+    }
+    if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Wavefront-per-quartet kernel (grid-stride over quartets), LDS-staged transforms.  FP64 only:
+// QCJK_MIXED leaves this variant's quartets in FP64.
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
+{
+    using S = HeavyShape<LA, LB, LC, LD>;
+    using Dg = HeavyDigest<LA, LB, LC, LD>;
+    __shared__ HeavySmem<LA, LB, LC, LD> sm;
+    const int lane = threadIdx.x;
+    HeavyLane<LA, LB, LC, LD> ln;
+    ln.init(lane);
+    const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
+        const int P = find_unit_owner(C.unit_start, C.nbra, u);
+        const int Q = (int)(u - C.unit_start[P]);
+        const int4 pi = A.pinfo[C.bra_off + P];
+        const double4 pg = A.pgeo[C.bra_off + P];
+        const int4 qi = A.pinfo[C.ket_off + Q];
+        const double4 qg = A.pgeo[C.ket_off + Q];
+        if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
+        heavy_compute<LA, LB, LC, LD>(A, sm, ln, lane, pi, pg, qi, qg);
+        const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+        for (int t = lane; t < Dg::NTASK; t += 64) {
+            int row, col;
+            double val;
+            if (Dg::task(t, sm.XI, A.D, A.n, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
+                double* tgt = t < Dg::T2 ? A.Jt : A.Kt;
+                unsafeAtomicAdd(&tgt[row * A.n + col], val);
+            }
+        }
+        if (A.counters && lane == 0) {
+            atomicAdd(&A.counters[0], 1ull);
+            atomicAdd(&A.counters[2], (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
+{
+    using Dg = HeavyDigest<LA, LB, LC, LD>;
+    using S = HeavyShape<LA, LB, LC, LD>;
+    __shared__ int rtab[S::NR];
+    __shared__ double sI[S::NI];
+    __shared__ H2Union<LA, LB, LC, LD> sm;
+    const int lane = threadIdx.x;
+    heavy2_init<LA, LB, LC, LD>(rtab, lane);
+    int t0 = 0, u0 = 0, v0 = 0;
+    if (lane < S::NHAB) hdec(lane, t0, u0, v0);
+    const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
+        const int P = find_unit_owner(C.unit_start, C.nbra, u);
+        const int Q = (int)(u - C.unit_start[P]);
+        const int4 pi = A.pinfo[C.bra_off + P];
+        const double4 pg = A.pgeo[C.bra_off + P];
+        const int4 qi = A.pinfo[C.ket_off + Q];
+        const double4 qg = A.pgeo[C.ket_off + Q];
+        if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
+        const bool fp32 = (A.flags & kFlagMixed) && pg.w * qg.w * (*A.dmax) < A.mixed_tol;  // one quartet per wave
+        if (fp32) heavy2_compute<LA, LB, LC, LD, float>(A, sm.f, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        else heavy2_compute<LA, LB, LC, LD, double>(A, sm.d, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+        for (int t = lane; t < Dg::NTASK; t += 64) {
+            int row, col;
+            double val;
+            if (Dg::task(t, sI, A.D, A.n, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
+                double* tgt = t < Dg::T2 ? A.Jt : A.Kt;
+                unsafeAtomicAdd(&tgt[row * A.n + col], val);
+            }
+        }
+        if (A.counters && lane == 0) {
+            atomicAdd(&A.counters[0], 1ull);
+            if (fp32) atomicAdd(&A.counters[1], 1ull);
+            atomicAdd(&A.counters[2], (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
+        }
+        __syncthreads();  // digest reads of sI finish before the next quartet clears it
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers + dispatch table
+// ---------------------------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_light(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    if (c.nunits <= 0) return hipSuccess;
+    const long long blocks = (c.nunits + 3) / 4;
+    hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_lightc(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    if (c.nunits <= 0) return hipSuccess;
+    const long long blocks = (c.nunits + 3) / 4;
+    hipLaunchKernelGGL((k_jk_lightc<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    if (c.nunits <= 0) return hipSuccess;
+    const long long cap = 256LL * 64;  // 64 waves per CU of 256 CUs in flight at most
+    const long long blocks = c.nunits < cap ? c.nunits : cap;
+    hipLaunchKernelGGL((k_jk_heavy<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    if (c.nunits <= 0) return hipSuccess;
+    const long long cap = 256LL * 64;
+    const long long blocks = c.nunits < cap ? c.nunits : cap;
+    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    return hipGetLastError();
+}
+
+}  // namespace qcjk

@@ -93,6 +93,8 @@ int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const dou
         res->t_jk = r.t_jk;
         res->t_diag = r.t_diag;
         res->t_setup = r.t_setup;
+        res->quartets = r.quartets;
+        res->quartets_fp32 = r.quartets_fp32;
         if (D_out && r.D.n == b.nbf) std::memcpy(D_out, r.D.a.data(), r.D.a.size() * sizeof(double));
         for (size_t i = 0; i < r.energy_history.size() && (int)i < par->max_iter; ++i) {
             if (e_hist) e_hist[i] = r.energy_history[i];

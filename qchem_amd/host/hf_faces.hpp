@@ -65,6 +65,9 @@ public:
     qcjk_handle* handle() const { return h_; }
     int nbf() const { return n_; }
     const qcjk_stats& last_stats() const { return stats_; }
+    // Shell quartets computed over every build so far, and those evaluated in FP32.
+    long long quartets_total() const { return quartets_; }
+    long long quartets_fp32_total() const { return quartets_fp32_; }
     int builds() const { return nbuilds_; }
     void set_flags(unsigned f) { flags_ = f; memo_valid_ = false; }
 
@@ -121,6 +124,8 @@ private:
         memoD_ = D.a;
         memo_valid_ = true;
         ++nbuilds_;
+        quartets_ += stats_.quartets_computed;
+        quartets_fp32_ += stats_.quartets_fp32;
     }
 
     qcjk_handle* h_ = nullptr;
@@ -131,6 +136,7 @@ private:
     mutable Mat J_, K_;
     mutable qcjk_stats stats_{};
     mutable int nbuilds_ = 0;
+    mutable long long quartets_ = 0, quartets_fp32_ = 0;
 };
 
 }  // namespace qchem_amd

@@ -218,6 +218,8 @@ SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const S
         res.D = D;
     }
     res.t_total = secs(t_start, clk::now());
+    res.quartets = eng.quartets_total();
+    res.quartets_fp32 = eng.quartets_fp32_total();
     return res;
 }
 

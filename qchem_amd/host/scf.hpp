@@ -44,6 +44,7 @@ struct SCFResult {
     std::vector<double> energy_history;  // E after each Fock build
     std::vector<double> time_history;    // seconds since start at each E
     double t_total = 0, t_jk = 0, t_diag = 0, t_setup = 0;
+    long long quartets = 0, quartets_fp32 = 0;  // over all Fock builds of this run
 };
 
 double nuclear_repulsion(const std::vector<Atom>& atoms);

@@ -1,0 +1,109 @@
+"""GPU: mixed-precision J/K (QCJK_MIXED).
+
+Quartets whose Schwarz estimate Q_ab Q_cd max|D'| is below mixed_tol evaluate their primitive
+integrals in FP32 (the Boys function at T >= 40 stays FP64); the contracted block is digested into
+J/K in FP64.  Every result is checked against the FP64 oracle:
+
+* at mixed_tol = 1e-7 (the production threshold) the J/K error stays below 1e-9 max|ref| and the
+  SCF energy within 1e-6 Eh of the FP64 one;
+* at mixed_tol = 1e30 every quartet of the variant runs in FP32, so each class's FP32 kernel is
+  exercised; the bar is then FP32 rounding, 1e-5 max|ref|.
+All calls go through the C ABI (libqcjk.so) via qchem_amd; the oracle is the checker only.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import qchem_amd as qa
+from qchem_amd import systems as S
+
+from test_gpu_jk import rand_density, small_cluster
+
+pytestmark = pytest.mark.gpu
+
+CLASSES = [(a, b, c, d) for a in range(3) for b in range(a + 1) for c in range(3) for d in range(c + 1)
+           if a * 3 + b >= c * 3 + d]
+
+
+def abs_err(x, ref):
+    return float(np.abs(x - ref).max() / np.abs(ref).max())
+
+
+def test_mixed_threshold_error_bound(require_gpu):
+    m = small_cluster(6, 71)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 71)
+    Jr, Kr, _ = oracle.Basis(m["shells"]).jk_direct(D)
+    eng = qa.JKEngine(B, mixed_tol=1e-7)
+    J64, K64, st64 = eng.build(D)
+    J, K, st = eng.build(D, flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED)
+    eJ, eK = abs_err(J, Jr), abs_err(K, Kr)
+    print("mixed 1e-7: fp32 quartets %d of %d, J err %.2e K err %.2e (fp64 %.2e %.2e)" % (
+        st.quartets_fp32, st.quartets_computed, eJ, eK, abs_err(J64, Jr), abs_err(K64, Kr)))
+    assert st64.quartets_fp32 == 0
+    assert st.quartets_computed == st64.quartets_computed
+    assert st.quartets_fp32 > 0.05 * st.quartets_computed
+    assert eJ < 1e-9 and eK < 1e-9
+
+
+def test_mixed_zero_tol_is_fp64(require_gpu):
+    m = small_cluster(4, 72)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 72)
+    eng = qa.JKEngine(B, mixed_tol=0.0)
+    J0, K0, _ = eng.build(D)
+    J1, K1, st = eng.build(D, flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED)
+    assert st.quartets_fp32 == 0
+    assert abs_err(J1, J0) < 1e-13 and abs_err(K1, K0) < 1e-13
+
+
+@pytest.mark.parametrize("variant", [0, 2, 3])
+def test_fp32_kernels_every_class(require_gpu, variant):
+    """mixed_tol = 1e30 sends every quartet of the chosen variant to its FP32 path."""
+    m = small_cluster(5, 51)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 51)
+    Jr, Kr, _ = oracle.Basis(m["shells"]).jk_direct(D)
+    eng = qa.JKEngine(B, mixed_tol=1e30)
+    used = []
+    for cls in CLASSES:
+        try:
+            eng.set_variant(*cls, variant)
+            used.append(cls)
+        except qa.EngineError as e:
+            assert e.code == 2 and variant in (0, 3)
+    J, K, st = eng.build(D, flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED)
+    frac = st.quartets_fp32 / st.quartets_computed
+    eJ, eK = abs_err(J, Jr), abs_err(K, Kr)
+    print("variant %d (%d classes): fp32 fraction %.3f, J err %.2e K err %.2e" % (variant, len(used), frac, eJ, eK))
+    assert st.quartets_fp32 > 0
+    if variant == 2:
+        assert st.quartets_fp32 == st.quartets_computed
+    assert eJ < 1e-5 and eK < 1e-5
+    assert eJ > 1e-12  # the FP32 path really ran
+
+
+def test_heavy_v1_stays_fp64(require_gpu):
+    m = small_cluster(3, 73)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 73)
+    Jr, Kr, _ = oracle.Basis(m["shells"]).jk_direct(D)
+    eng = qa.JKEngine(B, mixed_tol=1e30)
+    for cls in CLASSES:
+        eng.set_variant(*cls, 1)
+    J, K, st = eng.build(D, flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED)
+    assert st.quartets_fp32 == 0
+    assert abs_err(J, Jr) < 1e-12 and abs_err(K, Kr) < 1e-12
+
+
+def test_mixed_rhf_energy(require_gpu):
+    m = S.acetone_like()
+    B = qa.Basis(m["shells"])
+    r64 = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7)
+    rmx = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7, jk_flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED,
+                 mixed_tol=1e-7)
+    print("RHF fp64 %.10f (%d it) mixed %.10f (%d it) dE %.2e" % (
+        r64["energy"], r64["iterations"], rmx["energy"], rmx["iterations"], rmx["energy"] - r64["energy"]))
+    assert r64["converged"] and rmx["converged"]
+    assert rmx["fp32_fraction"] > 0.05
+    assert abs(rmx["energy"] - r64["energy"]) < 1e-6
