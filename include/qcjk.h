@@ -71,7 +71,8 @@ enum {
     QCJK_J = 1u,       /* build J */
     QCJK_K = 2u,       /* build K */
     QCJK_MIXED = 4u,   /* FP32 for quartets below opts.mixed_tol */
-    QCJK_SCREEN = 8u   /* Schwarz screening at opts.screen_tol */
+    QCJK_SCREEN = 8u,  /* Schwarz screening at opts.screen_tol */
+    QCJK_GRAPH = 16u   /* qcjk_build only: replay the captured build (qcjk_graph_build) */
 };
 
 enum {
@@ -91,10 +92,22 @@ enum {
  * ERI4 materialisation (IrrepBasisSet.C:138-183) -- nothing O(N^4) is ever stored. */
 int qcjk_create(const qcjk_basis* basis, const qcjk_opts* opts, qcjk_handle** out);
 
-/* Number of basis functions n. */
+/* Number of basis functions n (over all molecules of a batch handle). */
 int qcjk_nbf(const qcjk_handle* h);
 
-/* J and/or K from a HOST density (n x n).  Requested outputs are OVERWRITTEN; an output that is null
+/* Batch of independent molecules in one handle (no reference counterpart: the reference runs one
+ * molecule per process).  Each molecule keeps its own basis, shell pairs and n_m x n_m matrices;
+ * quartets never mix molecules.  Every build runs ONE launch per shell-quartet class over all
+ * molecules.  D, J and K of a batch handle are PACKED: molecule m's row-major block starts at
+ * element offset[m] (qcjk_layout) and the arrays hold nelem = sum n_m^2 elements; for one molecule
+ * this is the plain n x n matrix.  Screening and the FP32 choice use max|D| over the batch. */
+int qcjk_create_batch(const qcjk_basis* mols, int nmol, const qcjk_opts* opts, qcjk_handle** out);
+
+/* Packed layout: molecule count, n_m and element offset per molecule ([nmol] arrays, nullable),
+ * total elements.  */
+int qcjk_layout(const qcjk_handle* h, int32_t* nmol, int32_t* nbf, int64_t* offset, int64_t* nelem);
+
+/* J and/or K from a HOST density (n x n; packed blocks for a batch handle).  Requested outputs are OVERWRITTEN; an output that is null
  * or not requested in flags is not written.
  * Replaces MakeAOFock(Dao, exchange) (Orbital_HF_IBS.C:84) for both exchange=false/true at once. */
 int qcjk_build(qcjk_handle* h, const double* D, double* J, double* K, unsigned flags, qcjk_stats* stats);
@@ -104,11 +117,25 @@ int qcjk_build(qcjk_handle* h, const double* D, double* J, double* K, unsigned f
 int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags,
                       void* stream, qcjk_stats* stats);
 
+/* The per-iteration build captured once as a hipGraph and replayed (no reference counterpart; for
+ * SCF loops, whose every iteration repeats the same launches).  Same arguments and results as
+ * qcjk_build_device without stats; the graph is re-captured when the pointers, the flags or a
+ * class's kernel variant change.  Not while per-class profiling is enabled (QCJK_ERR_ARG). */
+int qcjk_graph_build(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags, void* stream);
+
+/* The handle's own device buffers for D, J and K (nelem doubles each), e.g. for qcjk_graph_build. */
+int qcjk_buffers(qcjk_handle* h, double** dD, double** dJ, double** dK);
+
 /* One-electron matrices on the device: S, T = 1/2 <-nabla^2>, V = sum_A -Z_A <a|1/|r-R_A||b>
  * (host pointers, n x n, overwritten).  Replaces the Overlap2C/Grad2/Nuclear kernels
  * (PG_Cart_MnD/Imp/GaussianRF.C:335-405) and Kinetic.C:57.  Any output may be null. */
 int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* R,
                       double* S, double* T, double* V);
+
+/* Same for molecule mol of a batch handle (n_m x n_m outputs).  qcjk_one_electron is mol 0 of a
+ * one-molecule handle and returns QCJK_ERR_ARG on a batch handle. */
+int qcjk_one_electron_mol(qcjk_handle* h, int mol, int natom, const double* Z, const double* R,
+                          double* S, double* T, double* V);
 
 /* Schwarz bounds Q_AB = sqrt(max |(ab|ab)|) (normalised functions), nshell x nshell, host. */
 int qcjk_schwarz(qcjk_handle* h, double* Q);

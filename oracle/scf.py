@@ -8,6 +8,10 @@ Follows the reference recipe:
 * DIIS: SCFAcceleratorDIIS.C:35-212 and Math/DIIS.C:24-51.  The error is [F', D'] in an
   orthonormal basis.  DIIS engages only once ||E|| < FDMax = Z^2 * 0.1 / 32 (Calculation.C:184),
   uses Nproj = 4, and prunes the oldest entry while svmin(B_bordered) < SVTol.
+* Density mixing: the LinearMixer (ChargeDensity/DensityMixer.C:104-139) as the fixed-point loop
+  drives it (SCFIterator.C:265-351): D_in <- relax D_out + (1-relax) D_in; when ||[F,D]|| rose
+  since the last iteration the step is re-damped to relax/4 and relax *= 0.8; when it fell relax
+  *= 1.5, capped at RelMax (1, or 0.5 once ||D_out - D_in||_F / N < 1e-5).  StartingRelaxRo = 1.
 * Loop: SCFIterator.C:240-362.
 
 J/K come from the C oracle (reference algorithm or direct).  Diagonalisation uses numpy.
@@ -36,6 +40,27 @@ def _density(X, Fp, nocc):
     return 2.0 * C @ C.T, e
 
 
+class LinearMixer:
+    """DensityMixer.C:104-139 (relax0 = StartingRelaxRo, SCFParams.C:32)."""
+
+    def __init__(self, relax0=1.0):
+        self.relax, self.relmax = relax0, 1.0
+
+    def step(self, D_in, D_out, fd, fd_old, nelec):
+        dcd = np.linalg.norm(D_out - D_in) / nelec
+        if dcd < 1e-5:
+            self.relmax = 0.5
+        if fd > fd_old and abs(fd - fd_old) > 1e-9:  # WantsReDamp -> ReDampMix
+            r = self.relax / 4.0
+            self.relax *= 0.8
+        else:
+            r = self.relax
+        if fd < fd_old:  # UpdateRelax
+            self.relax *= 1.5
+        self.relax = min(self.relax, self.relmax)
+        return r * D_out + (1.0 - r) * D_in
+
+
 def rhf(basis, Z, R, jk, charge=0, max_iter=100, e_tol=1e-10, fd_tol=1e-7, nproj=4, sv_tol=5e-9, D0=None):
     """basis: oracle.Basis; jk(D) -> (J, K).  Returns dict(energy, D, iterations, history)."""
     S, T, V = basis.one_electron(Z, R)
@@ -49,6 +74,8 @@ def rhf(basis, Z, R, jk, charge=0, max_iter=100, e_tol=1e-10, fd_tol=1e-7, nproj
     D = D0 if D0 is not None else _density(X, X.T @ h @ X, nocc)[0]
     Fs, Es = [], []
     E_old = None
+    mixer = LinearMixer()
+    fd_old = 1e10
     hist = []
     times = []
     import time as _time
@@ -91,5 +118,6 @@ def rhf(basis, Z, R, jk, charge=0, max_iter=100, e_tol=1e-10, fd_tol=1e-7, nproj
                     c = np.linalg.solve(B, rhs)[:N]
                     Fx = sum(ci * f for ci, f in zip(c, Fs))
                 break
-        D = _density(X, Fx, nocc)[0]
+        D = mixer.step(D, _density(X, Fx, nocc)[0], fd, fd_old, float(ztot - charge))
+        fd_old = fd
     return dict(energy=E, D=D, iterations=max_iter, history=hist, times=times, converged=False)

@@ -86,10 +86,17 @@ class _ScfResult(ctypes.Structure):
                 ("quartets", ctypes.c_int64), ("quartets_fp32", ctypes.c_int64)]
 
 
-QCJK_SYMBOLS = ["qcjk_create", "qcjk_nbf", "qcjk_build", "qcjk_build_device", "qcjk_one_electron",
-                "qcjk_schwarz", "qcjk_profile", "qcjk_class_report", "qcjk_set_variant", "qcjk_autotune",
-                "qcjk_destroy", "qcjk_strerror"]
-HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qchost_last_error"]
+QCJK_SYMBOLS = ["qcjk_create", "qcjk_create_batch", "qcjk_nbf", "qcjk_layout", "qcjk_build", "qcjk_build_device",
+                "qcjk_graph_build", "qcjk_buffers", "qcjk_one_electron", "qcjk_one_electron_mol", "qcjk_schwarz",
+                "qcjk_profile", "qcjk_class_report", "qcjk_set_variant", "qcjk_autotune", "qcjk_destroy",
+                "qcjk_strerror"]
+HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qcscf_rhf_batch", "qchost_last_error"]
+
+
+class _BatchResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("nconverged", ctypes.c_int32), ("t_total", ctypes.c_double),
+                ("t_setup", ctypes.c_double), ("t_jk", ctypes.c_double), ("t_host", ctypes.c_double),
+                ("jk_device_ms", ctypes.c_double), ("quartets", ctypes.c_int64), ("quartets_fp32", ctypes.c_int64)]
 
 
 def lib_paths():
@@ -114,7 +121,15 @@ def lib():
         L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         L.qcjk_create.argtypes = [ctypes.POINTER(_Basis), ctypes.POINTER(_Opts), ctypes.POINTER(vp)]
+        L.qcjk_create_batch.argtypes = [ctypes.POINTER(_Basis), ctypes.c_int, ctypes.POINTER(_Opts), ctypes.POINTER(vp)]
         L.qcjk_nbf.argtypes = [vp]
+        L.qcjk_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.qcjk_graph_build.argtypes = [vp, vp, vp, vp, ctypes.c_uint, vp]
+        L.qcjk_buffers.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        L.qcjk_one_electron_mol.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.qcjk_build.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double), ctypes.c_uint, ctypes.POINTER(Stats)]
         L.qcjk_build_device.argtypes = [vp, vp, vp, vp, ctypes.c_uint, vp, ctypes.POINTER(Stats)]
@@ -149,6 +164,11 @@ def host_lib():
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_ScfResult),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                 ctypes.POINTER(ctypes.c_double)]
+        H.qcscf_rhf_batch.argtypes = [ctypes.POINTER(_Shells), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_ScfParams), ctypes.c_int32,
+                                      ctypes.POINTER(_ScfResult), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(_BatchResult)]
         H.qchost_last_error.restype = ctypes.c_char_p
         _HOST = H
     return _HOST
@@ -190,17 +210,36 @@ class Basis:
 
 
 class JKEngine:
-    """One engine handle (one HIP stream).  ``build`` overwrites J and K for a symmetric D."""
+    """One engine handle (one HIP stream).  ``build`` overwrites J and K for a symmetric D.
+
+    ``basis`` is one Basis, or a list of Bases for a batch handle (qcjk_create_batch): then D, J and
+    K are lists of per-molecule matrices on the host, or packed device arrays of ``nelem`` doubles.
+    """
 
     def __init__(self, basis, device=0, screen_tol=0.0, mixed_tol=0.0):
-        self.basis = basis
-        self.n = basis.nbf
+        self.batch = isinstance(basis, (list, tuple))
+        self.bases = list(basis) if self.batch else [basis]
+        self.basis = self.bases[0]
         opts = _Opts(device, float(screen_tol), float(mixed_tol))
         h = ctypes.c_void_p()
-        rc = lib().qcjk_create(ctypes.byref(basis._cbasis()), ctypes.byref(opts), ctypes.byref(h))
+        if self.batch:
+            arr = (_Basis * len(self.bases))(*[b._cbasis() for b in self.bases])
+            rc = lib().qcjk_create_batch(arr, len(self.bases), ctypes.byref(opts), ctypes.byref(h))
+        else:
+            rc = lib().qcjk_create(ctypes.byref(basis._cbasis()), ctypes.byref(opts), ctypes.byref(h))
         if rc != 0:
-            raise EngineError("qcjk_create", rc)
+            raise EngineError("qcjk_create_batch" if self.batch else "qcjk_create", rc)
         self.h = h
+        nm = len(self.bases)
+        nbf = np.zeros(nm, dtype=np.int32)
+        off = np.zeros(nm, dtype=np.int64)
+        nmol, nelem = ctypes.c_int32(0), ctypes.c_int64(0)
+        rc = lib().qcjk_layout(h, ctypes.byref(nmol), _ip(nbf), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                               ctypes.byref(nelem))
+        if rc != 0:
+            raise EngineError("qcjk_layout", rc)
+        self.nbfs, self.offsets, self.nelem = nbf.tolist(), off.tolist(), int(nelem.value)
+        self.n = lib().qcjk_nbf(h)
 
     def close(self):
         if getattr(self, "h", None):
@@ -213,35 +252,70 @@ class JKEngine:
         except Exception:
             pass
 
+    def pack(self, mats):
+        flat = np.zeros(self.nelem)
+        for M, o, n in zip(mats, self.offsets, self.nbfs):
+            assert M.shape == (n, n)
+            flat[o:o + n * n] = np.asarray(M, dtype=np.float64).reshape(-1)
+        return flat
+
+    def unpack(self, flat):
+        return [np.asarray(flat[o:o + n * n]).reshape(n, n) for o, n in zip(self.offsets, self.nbfs)]
+
     def build(self, D, flags=QCJK_J | QCJK_K):
-        D = np.ascontiguousarray(D, dtype=np.float64)
-        assert D.shape == (self.n, self.n)
-        J = np.zeros_like(D)
-        K = np.zeros_like(D)
+        flat = self.pack(D) if self.batch else np.ascontiguousarray(D, dtype=np.float64)
+        if not self.batch:
+            assert flat.shape == (self.n, self.n)
+        J = np.zeros(self.nelem)
+        K = np.zeros(self.nelem)
         st = Stats()
-        rc = lib().qcjk_build(self.h, _dp(D), _dp(J), _dp(K), flags, ctypes.byref(st))
+        rc = lib().qcjk_build(self.h, _dp(flat), _dp(J), _dp(K), flags, ctypes.byref(st))
         if rc != 0:
             raise EngineError("qcjk_build", rc)
-        return J, K, st
+        if self.batch:
+            return self.unpack(J), self.unpack(K), st
+        return J.reshape(self.n, self.n), K.reshape(self.n, self.n), st
+
+    @staticmethod
+    def _ptr(x):
+        return None if x is None else (x if isinstance(x, int) else x.data_ptr())
+
+    @staticmethod
+    def _stream(stream):
+        return None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
 
     def build_device(self, dD, dJ, dK, flags=QCJK_J | QCJK_K, stream=None, stats=False):
         """Device pointers (ints) or torch CUDA tensors; asynchronous on ``stream`` unless stats."""
-        def ptr(x):
-            return None if x is None else (x if isinstance(x, int) else x.data_ptr())
         st = Stats() if stats else None
-        s = None if stream is None else (stream if isinstance(stream, int) else stream.cuda_stream)
-        rc = lib().qcjk_build_device(self.h, ptr(dD), ptr(dJ), ptr(dK), flags, s,
-                                     ctypes.byref(st) if st is not None else None)
+        rc = lib().qcjk_build_device(self.h, self._ptr(dD), self._ptr(dJ), self._ptr(dK), flags,
+                                     self._stream(stream), ctypes.byref(st) if st is not None else None)
         if rc != 0:
             raise EngineError("qcjk_build_device", rc)
         return st
 
-    def one_electron(self, Z, R):
+    def graph_build(self, dD, dJ, dK, flags=QCJK_J | QCJK_K, stream=None):
+        """The same build replayed from a captured hipGraph (captured on first use)."""
+        rc = lib().qcjk_graph_build(self.h, self._ptr(dD), self._ptr(dJ), self._ptr(dK), flags, self._stream(stream))
+        if rc != 0:
+            raise EngineError("qcjk_graph_build", rc)
+
+    def buffers(self):
+        """Device addresses of the handle's own D, J, K buffers (nelem doubles each)."""
+        p = [ctypes.c_void_p() for _ in range(3)]
+        rc = lib().qcjk_buffers(self.h, *[ctypes.byref(x) for x in p])
+        if rc != 0:
+            raise EngineError("qcjk_buffers", rc)
+        return [x.value for x in p]
+
+    def one_electron(self, Z, R, mol=None):
         Z = np.ascontiguousarray(Z, dtype=np.float64)
         R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
-        n = self.n
+        n = self.nbfs[mol or 0]
         S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
-        rc = lib().qcjk_one_electron(self.h, len(Z), _dp(Z), _dp(R), _dp(S), _dp(T), _dp(V))
+        if mol is None:
+            rc = lib().qcjk_one_electron(self.h, len(Z), _dp(Z), _dp(R), _dp(S), _dp(T), _dp(V))
+        else:
+            rc = lib().qcjk_one_electron_mol(self.h, mol, len(Z), _dp(Z), _dp(R), _dp(S), _dp(T), _dp(V))
         if rc != 0:
             raise EngineError("qcjk_one_electron", rc)
         return S, T, V
@@ -314,3 +388,31 @@ def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, f
             "energy_history": eh[:it].copy(), "time_history": th[:it].copy(), "t_total": res.t_total,
             "t_jk": res.t_jk, "t_diag": res.t_diag, "t_setup": res.t_setup, "quartets": res.quartets,
             "fp32_fraction": res.quartets_fp32 / max(1, res.quartets)}
+
+
+def rhf_batch(bases, mols, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_max=-1.0, sv_tol=5e-9,
+              jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, nthreads=16):
+    """Batched closed-shell RHF (qcscf_rhf_batch): one batched J/K build per iteration for every
+    molecule.  ``mols``: dicts with Z, R (and optional charge).  Returns (per-molecule list, summary)."""
+    nmol = len(bases)
+    shells = (_Shells * nmol)(*[b._shells() for b in bases])
+    natom = np.array([len(m["Z"]) for m in mols], dtype=np.int32)
+    Z = np.ascontiguousarray(np.concatenate([np.asarray(m["Z"], dtype=np.float64) for m in mols]))
+    R = np.ascontiguousarray(np.concatenate([np.asarray(m["R"], dtype=np.float64).reshape(-1) for m in mols]))
+    charge = np.array([int(m.get("charge", 0)) for m in mols], dtype=np.int32)
+    par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device, 0)
+    res = (_ScfResult * nmol)()
+    eh = np.zeros(nmol * max_iter)
+    br = _BatchResult()
+    rc = host_lib().qcscf_rhf_batch(shells, nmol, _ip(natom), _dp(Z), _dp(R), _ip(charge), ctypes.byref(par),
+                                    int(nthreads), res, _dp(eh), ctypes.byref(br))
+    if rc != 0:
+        raise EngineError("qcscf_rhf_batch", rc, host_lib().qchost_last_error().decode())
+    out = []
+    for m in range(nmol):
+        r = res[m]
+        out.append({"energy": r.energy, "iterations": r.iterations, "converged": bool(r.converged), "nbf": r.nbf,
+                    "energy_history": eh[m * max_iter:m * max_iter + r.iterations].copy()})
+    summary = {k: getattr(br, k) for k, _ in _BatchResult._fields_}
+    return out, summary
+

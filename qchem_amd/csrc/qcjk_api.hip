@@ -1,12 +1,15 @@
 // qcjk_api.hip -- the C ABI declared in include/qcjk.h: handle life cycle, work lists, launches.
 //
 // Host-side responsibilities:
-//   * validate the flat basis, regenerate the Boys table (positive-term series in long double),
-//   * enumerate unique shell pairs (first shell has the higher angular momentum), group them by
-//     pair class, run the pair-precompute and Schwarz kernels, sort each class by the bound,
+//   * validate the flat basis (one molecule, or a batch of molecules sharing one handle), regenerate
+//     the Boys table (positive-term series in long double),
+//   * enumerate each molecule's unique shell pairs (first shell has the higher angular momentum),
+//     group them by pair class and molecule, run the pair-precompute and Schwarz kernels, sort each
+//     molecule's run of a class by the bound,
 //   * build, per (bra class >= ket class), the work-unit prefix sums the kernels binary-search,
-//   * per build: prescale D' = ns (x) ns o D, zero the accumulators, launch the 21 class kernels,
-//     symmetrise J = Jt + Jt^T (and K) with the ns post-scale.
+//   * per build: prescale D' = ns (x) ns o D, zero the accumulators, launch the 21 class kernels
+//     (one launch per class for the whole batch), symmetrise J = Jt + Jt^T (and K) with the ns
+//     post-scale; optionally as one captured hipGraph replayed per build.
 // No CPU fallback exists: every path needs the HIP device.
 #include "qcjk.h"
 #include "qcjk_internal.h"
@@ -40,7 +43,7 @@ struct ClassPairWork {
     long long nunits[2] = {0, 0};
     long long nquartets = 0;   // surviving at create time
     long long* d_start[2] = {nullptr, nullptr};
-    int* d_nsurv = nullptr;
+    int2* d_kseg = nullptr;
     double prim_flops = 0;     // per primitive quartet
     double digest_flops = 0;   // per contracted quartet
     bool fp32_ok = true;       // (2 alpha_max)^L of the class fits FP32 with headroom
@@ -82,17 +85,21 @@ hipError_t dalloc(T** p, size_t count)
     return hipMalloc((void**)p, count * sizeof(T));
 }
 
-__global__ void k_prescale(const double* __restrict__ D, const double* __restrict__ ns, int n, double* __restrict__ Dp,
-                           double* __restrict__ Jt, double* __restrict__ Kt, unsigned long long* __restrict__ dmax_bits)
+// Per molecule (blockIdx.y): mol = {element offset, n_m, first basis function, 0}.
+__global__ void k_prescale(const double* __restrict__ D, const double* __restrict__ ns, const int4* __restrict__ mol,
+                           double* __restrict__ Dp, double* __restrict__ Jt, double* __restrict__ Kt,
+                           unsigned long long* __restrict__ dmax_bits)
 {
+    const int4 M = mol[blockIdx.y];
+    const int n = M.y;
     const long long nn = (long long)n * n;
     double m = 0.0;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
         const int r = (int)(i / n), c = (int)(i % n);
-        const double v = D[i] * ns[r] * ns[c];
-        Dp[i] = v;
-        Jt[i] = 0.0;
-        Kt[i] = 0.0;
+        const double v = D[M.x + i] * ns[M.z + r] * ns[M.z + c];
+        Dp[M.x + i] = v;
+        Jt[M.x + i] = 0.0;
+        Kt[M.x + i] = 0.0;
         m = fmax(m, fabs(v));
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
@@ -106,15 +113,17 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 }
 
 __global__ void k_finalize(const double* __restrict__ Jt, const double* __restrict__ Kt, const double* __restrict__ ns,
-                           int n, double* __restrict__ J, double* __restrict__ K)
+                           const int4* __restrict__ mol, double* __restrict__ J, double* __restrict__ K)
 {
+    const int4 M = mol[blockIdx.y];
+    const int n = M.y;
     const long long nn = (long long)n * n;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
         const int r = (int)(i / n), c = (int)(i % n);
-        const long long t = (long long)c * n + r;
-        const double w = ns[r] * ns[c];
-        if (J) J[i] = (Jt[i] + Jt[t]) * w;
-        if (K) K[i] = (Kt[i] + Kt[t]) * w;
+        const long long t = M.x + (long long)c * n + r;
+        const double w = ns[M.z + r] * ns[M.z + c];
+        if (J) J[M.x + i] = (Jt[M.x + i] + Jt[t]) * w;
+        if (K) K[M.x + i] = (Kt[M.x + i] + Kt[t]) * w;
     }
 }
 
@@ -125,8 +134,13 @@ struct qcjk_handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     qcjk_opts opts{};
-    int n = 0, nshell = 0;
-    std::vector<int> sl, snp, spoff, sbf;
+    int n = 0, nshell = 0;           // basis functions and shells over all molecules
+    int nmol = 1;
+    long long nelem = 0;             // packed matrix elements, sum of n_m^2
+    std::vector<int> mol_shell0;     // [nmol+1] first shell of each molecule
+    std::vector<int4> mol;           // {element offset, n_m, first basis function, 0}
+    int max_nm = 0;
+    std::vector<int> sl, snp, spoff, sbf, smol;  // sbf: offset within the shell's molecule
     std::vector<double> scen, exps, coefs, ns;
     // pairs, sorted: class c occupies [coff[c], coff[c+1])
     int coff[kNumPairClasses + 1] = {};
@@ -141,6 +155,8 @@ struct qcjk_handle {
     double4 *d_pp0 = nullptr, *d_pp1 = nullptr;
     int4* d_pinfo = nullptr;
     double4* d_pgeo = nullptr;
+    int2* d_pmat = nullptr;
+    int4* d_mol = nullptr;
     double *d_D = nullptr, *d_Dp = nullptr, *d_Jt = nullptr, *d_Kt = nullptr, *d_J = nullptr, *d_K = nullptr;
     unsigned long long* d_dmax = nullptr;
     unsigned long long* d_counters = nullptr;  // 3 per class pair
@@ -151,6 +167,11 @@ struct qcjk_handle {
     std::vector<qcjk_class_stat> ptot;      // running totals
     bool pending = false;                   // events of the last build not yet folded into ptot
     unsigned pflags = 0;
+    // captured build (qcjk_graph_build): replayed while pointers, flags and variants are unchanged
+    hipGraphExec_t gexec = nullptr;
+    const double* gD = nullptr;
+    double *gJ = nullptr, *gK = nullptr;
+    unsigned gflags = 0;
 };
 
 #define HIPCHK(x)                                          \
@@ -164,9 +185,11 @@ static void free_handle(qcjk_handle* h)
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (auto& w : h->work) { hipFree(w.d_start[0]); hipFree(w.d_start[1]); hipFree(w.d_nsurv); }
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    for (auto& w : h->work) { hipFree(w.d_start[0]); hipFree(w.d_start[1]); hipFree(w.d_kseg); }
     void* ptrs[] = {h->d_boys, h->d_exps, h->d_coefs, h->d_cen, h->d_ns, h->d_spoff, h->d_snp, h->d_pp0, h->d_pp1,
-                    h->d_pinfo, h->d_pgeo, h->d_D, h->d_Dp, h->d_Jt, h->d_Kt, h->d_J, h->d_K, h->d_dmax, h->d_counters};
+                    h->d_pinfo, h->d_pgeo, h->d_pmat, h->d_mol, h->d_D, h->d_Dp, h->d_Jt, h->d_Kt, h->d_J, h->d_K,
+                    h->d_dmax, h->d_counters};
     for (void* p : ptrs) if (p) hipFree(p);
     for (hipEvent_t e : h->pev) hipEventDestroy(e);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -181,7 +204,7 @@ static BuildArgs make_args(qcjk_handle* h, unsigned flags)
     a.D = h->d_Dp;
     a.Jt = h->d_Jt;
     a.Kt = h->d_Kt;
-    a.n = h->n;
+    a.pmat = h->d_pmat;
     a.flags = flags;
     a.boys = h->d_boys;
     a.pp0 = h->d_pp0;
@@ -195,7 +218,7 @@ static BuildArgs make_args(qcjk_handle* h, unsigned flags)
     return a;
 }
 
-static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
+static int create_impl(const qcjk_basis* mols, int nmol, const qcjk_opts* o, qcjk_handle* h)
 {
     h->opts = *o;
     h->device = o->device;
@@ -208,24 +231,42 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     HIPCHK(hipEventCreate(&h->ev1));
     const bool force_heavy = std::getenv("QCJK_FORCE_HEAVY") != nullptr;
 
-    const int ns = b->nshell;
-    h->nshell = ns;
-    h->sl.assign(b->shell_l, b->shell_l + ns);
-    h->snp.assign(b->shell_nprim, b->shell_nprim + ns);
-    h->spoff.resize(ns);
-    h->sbf.resize(ns);
-    int np = 0, nbf = 0;
-    for (int s = 0; s < ns; ++s) {
-        h->spoff[s] = np;
-        h->sbf[s] = nbf;
-        np += h->snp[s];
-        nbf += ncart(h->sl[s]);
+    // concatenate the molecules' shells; basis functions are numbered within each molecule
+    h->nmol = nmol;
+    h->mol_shell0.assign(nmol + 1, 0);
+    h->mol.resize(nmol);
+    int np = 0, nbf = 0, ns = 0;
+    long long nelem = 0;
+    for (int m = 0; m < nmol; ++m) {
+        const qcjk_basis* b = &mols[m];
+        h->mol_shell0[m] = ns;
+        int nm = 0, npm = 0;
+        for (int s = 0; s < b->nshell; ++s) {
+            h->sl.push_back(b->shell_l[s]);
+            h->snp.push_back(b->shell_nprim[s]);
+            h->spoff.push_back(np + npm);
+            h->sbf.push_back(nm);
+            h->smol.push_back(m);
+            for (int x = 0; x < 3; ++x) h->scen.push_back(b->shell_center[3 * s + x]);
+            npm += b->shell_nprim[s];
+            nm += ncart(b->shell_l[s]);
+        }
+        // exponents, coefficients and ns straight from the caller's arrays (shell order)
+        h->exps.insert(h->exps.end(), b->prim_exp, b->prim_exp + npm);
+        h->coefs.insert(h->coefs.end(), b->prim_coef, b->prim_coef + npm);
+        h->ns.insert(h->ns.end(), b->bf_norm, b->bf_norm + nm);
+        np += npm;
+        h->mol[m] = make_int4((int)nelem, nm, nbf, 0);
+        h->max_nm = std::max(h->max_nm, nm);
+        nelem += (long long)nm * nm;
+        nbf += nm;
+        ns += b->nshell;
     }
+    h->mol_shell0[nmol] = ns;
+    if (nelem > 0x7fffffffLL) return QCJK_ERR_UNSUPPORTED;  // element offsets are 32-bit
+    h->nshell = ns;
     h->n = nbf;
-    h->scen.assign(b->shell_center, b->shell_center + 3 * ns);
-    h->exps.assign(b->prim_exp, b->prim_exp + np);
-    h->coefs.assign(b->prim_coef, b->prim_coef + np);
-    h->ns.assign(b->bf_norm, b->bf_norm + nbf);
+    h->nelem = nelem;
     for (int i = 0; i < np; ++i)
         if (!(h->exps[i] > 0.0) || !std::isfinite(h->coefs[i])) return QCJK_ERR_ARG;
     for (int i = 0; i < nbf; ++i)
@@ -242,23 +283,27 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     HIPCHK(dalloc(&h->d_ns, nbf));
     HIPCHK(dalloc(&h->d_spoff, ns));
     HIPCHK(dalloc(&h->d_snp, ns));
+    HIPCHK(dalloc(&h->d_mol, nmol));
     HIPCHK(hipMemcpy(h->d_exps, h->exps.data(), np * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_coefs, h->coefs.data(), np * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_cen, h->scen.data(), 3 * ns * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_ns, h->ns.data(), nbf * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_spoff, h->spoff.data(), ns * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_snp, h->snp.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_mol, h->mol.data(), nmol * sizeof(int4), hipMemcpyHostToDevice));
 
-    // unique shell pairs, oriented (higher l first; equal l: higher index first), grouped by class
+    // unique shell pairs of each molecule, oriented (higher l first; equal l: higher index first),
+    // grouped by class, molecule order within a class
     std::vector<std::vector<int>> cls_first(kNumPairClasses), cls_second(kNumPairClasses);
-    for (int A = 0; A < ns; ++A)
-        for (int B = 0; B <= A; ++B) {
-            int f = A, s = B;
-            if (h->sl[A] < h->sl[B]) std::swap(f, s);
-            const int c = pair_class(h->sl[f], h->sl[s]);
-            cls_first[c].push_back(f);
-            cls_second[c].push_back(s);
-        }
+    for (int m = 0; m < nmol; ++m)
+        for (int A = h->mol_shell0[m]; A < h->mol_shell0[m + 1]; ++A)
+            for (int B = h->mol_shell0[m]; B <= A; ++B) {
+                int f = A, s = B;
+                if (h->sl[A] < h->sl[B]) std::swap(f, s);
+                const int c = pair_class(h->sl[f], h->sl[s]);
+                cls_first[c].push_back(f);
+                cls_second[c].push_back(s);
+            }
     std::vector<int> pf, psd;
     for (int c = 0; c < kNumPairClasses; ++c) {
         h->coff[c] = (int)pf.size();
@@ -299,11 +344,13 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     hipFree(d_pnp);
     hipFree(d_ppoff);
 
-    // unsorted pair records -> Schwarz
-    auto make_pinfo = [&](int k) {
+    // pair records.  The Schwarz kernels index ns by the basis-function offsets, so they get the
+    // global offsets; the J/K kernels get offsets within the molecule plus the molecule's block.
+    auto make_pinfo = [&](int k, bool global_bf) {
         const int f = pf[k], s = psd[k];
         const int npr = h->snp[f] * h->snp[s];
-        return make_int4(h->sbf[f], h->sbf[s], ppoff[k], npr | ((f == s ? 1 : 0) << 16));
+        const int base = global_bf ? h->mol[h->smol[f]].z : 0;
+        return make_int4(base + h->sbf[f], base + h->sbf[s], ppoff[k], npr | ((f == s ? 1 : 0) << 16));
     };
     auto make_pgeo = [&](int k, double q) {
         const int f = pf[k], s = psd[k];
@@ -314,11 +361,12 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     std::vector<double4> pgeo(npair);
     for (int k = 0; k < npair; ++k) {
         if (h->snp[pf[k]] * h->snp[psd[k]] > 0xffff) return QCJK_ERR_UNSUPPORTED;
-        pinfo[k] = make_pinfo(k);
+        pinfo[k] = make_pinfo(k, true);
         pgeo[k] = make_pgeo(k, 0.0);
     }
     HIPCHK(dalloc(&h->d_pinfo, npair));
     HIPCHK(dalloc(&h->d_pgeo, npair));
+    HIPCHK(dalloc(&h->d_pmat, npair));
     HIPCHK(hipMemcpy(h->d_pinfo, pinfo.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_pgeo, pgeo.data(), npair * sizeof(double4), hipMemcpyHostToDevice));
     double *d_qraw = nullptr, *d_qnorm = nullptr;
@@ -338,29 +386,44 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
     hipFree(d_qraw);
     hipFree(d_qnorm);
 
-    // sort each class by descending raw bound; rebuild records in sorted order
+    // sort each class by molecule, then descending raw bound; rebuild records in sorted order
     std::vector<int> order(npair);
     std::iota(order.begin(), order.end(), 0);
     for (int c = 0; c < kNumPairClasses; ++c)
-        std::stable_sort(order.begin() + h->coff[c], order.begin() + h->coff[c + 1],
-                         [&](int x, int y) { return qraw[x] > qraw[y]; });
+        std::stable_sort(order.begin() + h->coff[c], order.begin() + h->coff[c + 1], [&](int x, int y) {
+            const int mx = h->smol[pf[x]], my = h->smol[pf[y]];
+            return mx != my ? mx < my : qraw[x] > qraw[y];
+        });
     h->pair_first.resize(npair);
     h->pair_second.resize(npair);
     h->qraw.resize(npair);
     h->qnorm.resize(npair);
     h->pair_np.resize(npair);
+    std::vector<int2> pmat(npair);
+    std::vector<int> pmol(npair);
     for (int k = 0; k < npair; ++k) {
         const int src = order[k];
-        pinfo[k] = make_pinfo(src);
+        pinfo[k] = make_pinfo(src, false);
         pgeo[k] = make_pgeo(src, qraw[src]);
         h->pair_first[k] = pf[src];
         h->pair_second[k] = psd[src];
         h->qraw[k] = qraw[src];
         h->qnorm[k] = qnorm[src];
         h->pair_np[k] = h->snp[pf[src]] * h->snp[psd[src]];
+        pmol[k] = h->smol[pf[src]];
+        pmat[k] = make_int2(h->mol[pmol[k]].x, h->mol[pmol[k]].y);
     }
     HIPCHK(hipMemcpy(h->d_pinfo, pinfo.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_pgeo, pgeo.data(), npair * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_pmat, pmat.data(), npair * sizeof(int2), hipMemcpyHostToDevice));
+
+    // per class: each molecule's run [mstart[c][m], mstart[c][m+1]) (class-local indices)
+    std::vector<std::vector<int>> mstart(kNumPairClasses, std::vector<int>(nmol + 1, 0));
+    for (int c = 0; c < kNumPairClasses; ++c) {
+        std::vector<int>& st = mstart[c];
+        for (int k = h->coff[c]; k < h->coff[c + 1]; ++k) ++st[pmol[k] + 1];
+        for (int m = 0; m < nmol; ++m) st[m + 1] += st[m];
+    }
 
     // largest primitive-pair exponent p = a_i + b_j per pair class (FP32 range guard)
     double class_pmax[kNumPairClasses] = {0, 0, 0, 0, 0, 0};
@@ -374,7 +437,7 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
         class_pmax[c] = std::max(class_pmax[c], amax[0] + amax[1]);
     }
 
-    // work lists per (bra class >= ket class)
+    // work lists per (bra class >= ket class): each bra pair meets the ket pairs of its own molecule
     for (int bc = 0; bc < kNumPairClasses; ++bc)
         for (int kc = 0; kc <= bc; ++kc) {
             const int nb = h->coff[bc + 1] - h->coff[bc], nk = h->coff[kc + 1] - h->coff[kc];
@@ -394,36 +457,38 @@ static int create_impl(const qcjk_basis* b, const qcjk_opts* o, qcjk_handle* h)
             w.prim_flops = prim_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             w.digest_flops = digest_flops(w.info->la, w.info->lb, w.info->lc, w.info->ld);
             std::vector<long long> start0(nb + 1, 0), start1(nb + 1, 0);
-            std::vector<int> nsurv(nb, 0);
+            std::vector<int2> kseg(nb);
             const double* qk = h->qraw.data() + h->coff[kc];
             for (int P = 0; P < nb; ++P) {
+                const int m = pmol[h->coff[bc] + P];
+                const int k0 = mstart[kc][m], k1 = mstart[kc][m + 1];
                 const double qp = h->qraw[h->coff[bc] + P];
-                // ket list is sorted descending: survivors are a prefix
-                int lo = 0, hi = nk;
+                // the molecule's ket run is sorted descending: survivors are a prefix
+                int lo = k0, hi = k1;
                 while (lo < hi) {
                     const int mid = (lo + hi) / 2;
                     if (qp * qk[mid] >= kPruneTol) lo = mid + 1;
                     else hi = mid;
                 }
-                int m = lo;
-                if (bc == kc) m = std::min(m, P + 1);
-                nsurv[P] = m;
-                start0[P + 1] = start0[P] + (m + kTile[0] - 1) / kTile[0];
-                start1[P + 1] = start1[P] + m;
-                w.nquartets += m;
+                int cnt = lo - k0;
+                if (bc == kc) cnt = std::min(cnt, P - k0 + 1);  // ket index <= bra index
+                kseg[P] = make_int2(k0, std::max(cnt, 0));
+                start0[P + 1] = start0[P] + (kseg[P].y + kTile[0] - 1) / kTile[0];
+                start1[P + 1] = start1[P] + kseg[P].y;
+                w.nquartets += kseg[P].y;
             }
             w.nunits[0] = start0[nb];
             w.nunits[1] = start1[nb];
             HIPCHK(dalloc(&w.d_start[0], nb + 1));
             HIPCHK(dalloc(&w.d_start[1], nb + 1));
-            HIPCHK(dalloc(&w.d_nsurv, nb));
+            HIPCHK(dalloc(&w.d_kseg, nb));
             HIPCHK(hipMemcpy(w.d_start[0], start0.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(w.d_start[1], start1.data(), (nb + 1) * sizeof(long long), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(w.d_nsurv, nsurv.data(), nb * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(w.d_kseg, kseg.data(), nb * sizeof(int2), hipMemcpyHostToDevice));
             h->work.push_back(w);
         }
 
-    const size_t nn = (size_t)nbf * nbf;
+    const size_t nn = (size_t)nelem;
     HIPCHK(dalloc(&h->d_D, nn));
     HIPCHK(dalloc(&h->d_Dp, nn));
     HIPCHK(dalloc(&h->d_Jt, nn));
@@ -444,22 +509,30 @@ static ClassPairArgs class_args(const qcjk_handle* h, const ClassPairWork& w, in
     c.nket = h->coff[w.ket + 1] - h->coff[w.ket];
     c.same = w.bra == w.ket;
     c.unit_start = w.d_start[w.unit_list(variant)];
-    c.nsurv = w.d_nsurv;
+    c.kseg = w.d_kseg;
     c.nunits = w.nunits[w.unit_list(variant)];
     c.tile = kTile[variant];
     return c;
 }
 
 // Prescale D, zero the accumulators and counters (start of every build).
+// Elementwise kernels over the packed matrices: grid (x blocks per molecule, molecules).
+static dim3 elem_grid(const qcjk_handle* h)
+{
+    const long long per = ((long long)h->max_nm * h->max_nm + 255) / 256;
+    const long long cap = std::max<long long>(1, 4096 / h->nmol);
+    return dim3((unsigned)std::max<long long>(1, std::min(per, cap)), (unsigned)h->nmol);
+}
+
+// Prescale D, zero the accumulators and counters (start of every build).  Kernels only, so a
+// build can be captured into a hipGraph.
 static int begin_build(qcjk_handle* h, const double* dD, hipStream_t s)
 {
-    const int n = h->n;
-    const long long nn = (long long)n * n;
-    const int eb = (int)std::min<long long>((nn + 255) / 256, 4096);
     const int nctr = (int)(3 * h->work.size() + 3);
     hipLaunchKernelGGL(k_zero_u64, dim3((nctr + 255) / 256), dim3(256), 0, s, h->d_counters, nctr);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, h->d_dmax, 1);
-    hipLaunchKernelGGL(k_prescale, dim3(eb), dim3(256), 0, s, dD, h->d_ns, n, h->d_Dp, h->d_Jt, h->d_Kt, h->d_dmax);
+    hipLaunchKernelGGL(k_prescale, elem_grid(h), dim3(256), 0, s, dD, h->d_ns, h->d_mol, h->d_Dp, h->d_Jt, h->d_Kt,
+                       h->d_dmax);
     HIPCHK(hipGetLastError());
     return QCJK_OK;
 }
@@ -519,10 +592,8 @@ int qcjk_class_report(qcjk_handle* h, qcjk_class_stat* out, int max, int* count)
     return QCJK_OK;
 }
 
-int qcjk_create(const qcjk_basis* basis, const qcjk_opts* opts, qcjk_handle** out)
+static int check_basis(const qcjk_basis* basis)
 {
-    if (!out) return QCJK_ERR_ARG;
-    *out = nullptr;
     if (!basis || basis->nshell <= 0 || !basis->shell_l || !basis->shell_nprim || !basis->shell_center ||
         !basis->prim_exp || !basis->prim_coef || !basis->bf_norm)
         return QCJK_ERR_ARG;
@@ -531,11 +602,23 @@ int qcjk_create(const qcjk_basis* basis, const qcjk_opts* opts, qcjk_handle** ou
         if (basis->shell_l[s] < 0 || basis->shell_l[s] > QCJK_MAX_L) return QCJK_ERR_UNSUPPORTED;
         if (basis->shell_nprim[s] < 1 || basis->shell_nprim[s] > QCJK_MAX_NPRIM) return QCJK_ERR_UNSUPPORTED;
     }
+    return QCJK_OK;
+}
+
+int qcjk_create_batch(const qcjk_basis* mols, int nmol, const qcjk_opts* opts, qcjk_handle** out)
+{
+    if (!out) return QCJK_ERR_ARG;
+    *out = nullptr;
+    if (!mols || nmol < 1 || nmol > 65535) return QCJK_ERR_ARG;
+    for (int m = 0; m < nmol; ++m) {
+        const int rc = check_basis(&mols[m]);
+        if (rc != QCJK_OK) return rc;
+    }
     qcjk_opts o{};
     if (opts) o = *opts;
     qcjk_handle* h = new (std::nothrow) qcjk_handle();
     if (!h) return QCJK_ERR_NOMEM;
-    const int rc = create_impl(basis, &o, h);
+    const int rc = create_impl(mols, nmol, &o, h);
     if (rc != QCJK_OK) {
         free_handle(h);
         return rc;
@@ -544,25 +627,28 @@ int qcjk_create(const qcjk_basis* basis, const qcjk_opts* opts, qcjk_handle** ou
     return QCJK_OK;
 }
 
+int qcjk_create(const qcjk_basis* basis, const qcjk_opts* opts, qcjk_handle** out)
+{
+    return qcjk_create_batch(basis, 1, opts, out);
+}
+
 int qcjk_nbf(const qcjk_handle* h) { return h ? h->n : -1; }
 
-int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags, void* stream,
-                      qcjk_stats* stats)
+int qcjk_layout(const qcjk_handle* h, int32_t* nmol, int32_t* nbf, int64_t* offset, int64_t* nelem)
 {
-    if (!h || !dD) return QCJK_ERR_ARG;
-    if (!dJ) flags &= ~(unsigned)QCJK_J;
-    if (!dK) flags &= ~(unsigned)QCJK_K;
-    HIPCHK(hipSetDevice(h->device));
-    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-    const int n = h->n;
-    const long long nn = (long long)n * n;
-    const int eb = (int)std::min<long long>((nn + 255) / 256, 4096);
-    const int nctr = (int)(3 * h->work.size() + 3);
-    if (h->profiling) {
-        const int rc = fold_profile(h);
-        if (rc != QCJK_OK) return rc;
+    if (!h) return QCJK_ERR_ARG;
+    if (nmol) *nmol = h->nmol;
+    if (nelem) *nelem = h->nelem;
+    for (int m = 0; m < h->nmol; ++m) {
+        if (nbf) nbf[m] = h->mol[m].y;
+        if (offset) offset[m] = h->mol[m].x;
     }
-    if (stats) HIPCHK(hipEventRecord(h->ev0, s));
+    return QCJK_OK;
+}
+
+// Enqueue one build on s: prescale, the class launches, symmetrise.  Kernels only.
+static int enqueue_build(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags, hipStream_t s)
+{
     {
         const int rc = begin_build(h, dD, s);
         if (rc != QCJK_OK) return rc;
@@ -579,27 +665,100 @@ int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, 
             if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i + 1], s));
         }
     }
-    hipLaunchKernelGGL(k_finalize, dim3(eb), dim3(256), 0, s, h->d_Jt, h->d_Kt, h->d_ns, n,
+    hipLaunchKernelGGL(k_finalize, elem_grid(h), dim3(256), 0, s, h->d_Jt, h->d_Kt, h->d_ns, h->d_mol,
                        (flags & QCJK_J) ? dJ : nullptr, (flags & QCJK_K) ? dK : nullptr);
     HIPCHK(hipGetLastError());
+    return QCJK_OK;
+}
+
+static int read_stats(qcjk_handle* h, qcjk_stats* stats)
+{
+    const int nctr = (int)(3 * h->work.size() + 3);
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    std::vector<unsigned long long> ctr(nctr);
+    HIPCHK(hipMemcpy(ctr.data(), h->d_counters, nctr * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    std::memset(stats, 0, sizeof(*stats));
+    for (size_t i = 0; i < h->work.size(); ++i) {
+        const ClassPairWork& w = h->work[i];
+        stats->quartets_computed += (long long)ctr[3 * i];
+        stats->quartets_fp32 += (long long)ctr[3 * i + 1];
+        stats->prim_quartets += (long long)ctr[3 * i + 2];
+        stats->flops += (double)ctr[3 * i + 2] * w.prim_flops + (double)ctr[3 * i] * w.digest_flops;
+    }
+    // unique quartets of every molecule: sum over molecules of npairs_m (npairs_m + 1) / 2
+    std::vector<long long> np(h->nmol, 0);
+    for (long long k = 0; k < h->npairs; ++k) ++np[h->smol[h->pair_first[k]]];
+    for (int m = 0; m < h->nmol; ++m) stats->quartets_total += np[m] * (np[m] + 1) / 2;
+    stats->device_ms = ms;
+    return QCJK_OK;
+}
+
+int qcjk_build_device(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags, void* stream,
+                      qcjk_stats* stats)
+{
+    if (!h || !dD) return QCJK_ERR_ARG;
+    if (!dJ) flags &= ~(unsigned)QCJK_J;
+    if (!dK) flags &= ~(unsigned)QCJK_K;
+    HIPCHK(hipSetDevice(h->device));
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    if (h->profiling) {
+        const int rc = fold_profile(h);
+        if (rc != QCJK_OK) return rc;
+    }
+    if (stats) HIPCHK(hipEventRecord(h->ev0, s));
+    {
+        const int rc = enqueue_build(h, dD, dJ, dK, flags, s);
+        if (rc != QCJK_OK) return rc;
+    }
     if (h->profiling && (flags & (QCJK_J | QCJK_K))) h->pending = true;
     if (stats) {
         HIPCHK(hipEventRecord(h->ev1, s));
         HIPCHK(hipEventSynchronize(h->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-        std::vector<unsigned long long> ctr(nctr);
-        HIPCHK(hipMemcpy(ctr.data(), h->d_counters, nctr * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        std::memset(stats, 0, sizeof(*stats));
-        stats->quartets_total = h->npairs * (h->npairs + 1) / 2;
-        for (size_t i = 0; i < h->work.size(); ++i) {
-            stats->quartets_computed += (long long)ctr[3 * i];
-            stats->quartets_fp32 += (long long)ctr[3 * i + 1];
-            stats->prim_quartets += (long long)ctr[3 * i + 2];
-            stats->flops += (double)ctr[3 * i + 2] * h->work[i].prim_flops + (double)ctr[3 * i] * h->work[i].digest_flops;
-        }
-        stats->device_ms = ms;
+        return read_stats(h, stats);
     }
+    return QCJK_OK;
+}
+
+int qcjk_graph_build(qcjk_handle* h, const double* dD, double* dJ, double* dK, unsigned flags, void* stream)
+{
+    if (!h || !dD || h->profiling) return QCJK_ERR_ARG;
+    if (!dJ) flags &= ~(unsigned)QCJK_J;
+    if (!dK) flags &= ~(unsigned)QCJK_K;
+    HIPCHK(hipSetDevice(h->device));
+    if (!h->gexec || h->gD != dD || h->gJ != dJ || h->gK != dK || h->gflags != flags) {
+        if (h->gexec) {
+            HIPCHK(hipStreamSynchronize(h->stream));
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_build(h, dD, dJ, dK, flags, h->stream);
+        const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+        if (rc != QCJK_OK) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(ec);
+        const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        HIPCHK(ei);
+        h->gD = dD;
+        h->gJ = dJ;
+        h->gK = dK;
+        h->gflags = flags;
+    }
+    HIPCHK(hipGraphLaunch(h->gexec, stream ? (hipStream_t)stream : h->stream));
+    return QCJK_OK;
+}
+
+int qcjk_buffers(qcjk_handle* h, double** dD, double** dJ, double** dK)
+{
+    if (!h) return QCJK_ERR_ARG;
+    if (dD) *dD = h->d_D;
+    if (dJ) *dJ = h->d_J;
+    if (dK) *dK = h->d_K;
     return QCJK_OK;
 }
 
@@ -607,12 +766,26 @@ int qcjk_build(qcjk_handle* h, const double* D, double* J, double* K, unsigned f
 {
     if (!h || !D) return QCJK_ERR_ARG;
     HIPCHK(hipSetDevice(h->device));
-    const size_t bytes = (size_t)h->n * h->n * sizeof(double);
+    const size_t bytes = (size_t)h->nelem * sizeof(double);
     HIPCHK(hipMemcpyAsync(h->d_D, D, bytes, hipMemcpyHostToDevice, h->stream));
     const bool wantJ = J && (flags & QCJK_J), wantK = K && (flags & QCJK_K);
-    const int rc = qcjk_build_device(h, h->d_D, wantJ ? h->d_J : nullptr, wantK ? h->d_K : nullptr, flags, h->stream,
-                                     stats);
-    if (rc != QCJK_OK) return rc;
+    const bool graph = (flags & QCJK_GRAPH) != 0;
+    flags &= ~(unsigned)QCJK_GRAPH;
+    if (graph) {
+        if (stats) HIPCHK(hipEventRecord(h->ev0, h->stream));
+        const int rc = qcjk_graph_build(h, h->d_D, wantJ ? h->d_J : nullptr, wantK ? h->d_K : nullptr, flags, h->stream);
+        if (rc != QCJK_OK) return rc;
+        if (stats) {
+            HIPCHK(hipEventRecord(h->ev1, h->stream));
+            HIPCHK(hipEventSynchronize(h->ev1));
+            const int rs = read_stats(h, stats);
+            if (rs != QCJK_OK) return rs;
+        }
+    } else {
+        const int rc = qcjk_build_device(h, h->d_D, wantJ ? h->d_J : nullptr, wantK ? h->d_K : nullptr, flags, h->stream,
+                                         stats);
+        if (rc != QCJK_OK) return rc;
+    }
     // outputs not requested are left untouched
     if (wantJ) HIPCHK(hipMemcpyAsync(J, h->d_J, bytes, hipMemcpyDeviceToHost, h->stream));
     if (wantK) HIPCHK(hipMemcpyAsync(K, h->d_K, bytes, hipMemcpyDeviceToHost, h->stream));
@@ -620,12 +793,14 @@ int qcjk_build(qcjk_handle* h, const double* D, double* J, double* K, unsigned f
     return QCJK_OK;
 }
 
-int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* R, double* S, double* T, double* V)
+int qcjk_one_electron_mol(qcjk_handle* h, int mol, int natom, const double* Z, const double* R, double* S,
+                          double* T, double* V)
 {
-    if (!h || natom < 0 || (natom > 0 && (!Z || !R))) return QCJK_ERR_ARG;
+    if (!h || mol < 0 || mol >= h->nmol || natom < 0 || (natom > 0 && (!Z || !R))) return QCJK_ERR_ARG;
     HIPCHK(hipSetDevice(h->device));
-    const int n = h->n;
+    const int n = h->mol[mol].y;
     const size_t nn = (size_t)n * n;
+    const int s0 = h->mol_shell0[mol], s1 = h->mol_shell0[mol + 1];
     double *dZ = nullptr, *dR = nullptr, *dS = nullptr, *dT = nullptr, *dV = nullptr;
     int4* dsp = nullptr;
     int rc = QCJK_OK;
@@ -635,8 +810,8 @@ int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* 
     for (int la = 0; la <= QCJK_MAX_L; ++la)
         for (int lb = 0; lb <= la; ++lb) {
             const int start = (int)sp.size();
-            for (int A = 0; A < h->nshell; ++A)
-                for (int B = 0; B < h->nshell; ++B) {
+            for (int A = s0; A < s1; ++A)
+                for (int B = s0; B < s1; ++B) {
                     if (h->sl[A] != la || h->sl[B] != lb) continue;
                     if (la == lb && B > A) continue;
                     sp.push_back(make_int4(A, B, h->sbf[A], h->sbf[B]));
@@ -659,8 +834,8 @@ int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* 
         for (const OE& g : groups)
             if (rc == QCJK_OK)
                 fail(launch_one_electron(g.la, g.lb, dsp + g.start, g.count, h->d_spoff, h->d_snp, h->d_exps,
-                                         h->d_coefs, h->d_cen, h->d_ns, n, natom, dZ, dR, h->d_boys, dS, dT, dV,
-                                         h->stream));
+                                         h->d_coefs, h->d_cen, h->d_ns + h->mol[mol].z, n, natom, dZ, dR, h->d_boys,
+                                         dS, dT, dV, h->stream));
         fail(hipStreamSynchronize(h->stream));
         if (S) fail(hipMemcpy(S, dS, nn * sizeof(double), hipMemcpyDeviceToHost));
         if (T) fail(hipMemcpy(T, dT, nn * sizeof(double), hipMemcpyDeviceToHost));
@@ -668,6 +843,12 @@ int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* 
     }
     hipFree(dZ); hipFree(dR); hipFree(dS); hipFree(dT); hipFree(dV); hipFree(dsp);
     return rc;
+}
+
+int qcjk_one_electron(qcjk_handle* h, int natom, const double* Z, const double* R, double* S, double* T, double* V)
+{
+    if (!h || h->nmol != 1) return QCJK_ERR_ARG;
+    return qcjk_one_electron_mol(h, 0, natom, Z, R, S, T, V);
 }
 
 int qcjk_schwarz(qcjk_handle* h, double* Q)
@@ -690,6 +871,7 @@ int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant
     for (auto& w : h->work)
         if (w.info->la == la && w.info->lb == lb && w.info->lc == lc && w.info->ld == ld) {
             if (!w.has(variant)) return QCJK_ERR_UNSUPPORTED;
+            if (w.variant != variant) h->gD = nullptr;  // recapture the graph on its next use
             w.variant = variant;
             return QCJK_OK;
         }
@@ -730,6 +912,7 @@ int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* o
         double alt = 1e300;
         for (int v = 0; v < kNumVariants; ++v)
             if (v != bv) alt = std::min(alt, best[v]);
+        if (w.variant != bv) h->gD = nullptr;  // recapture the graph on its next use
         w.variant = bv;
         qcjk_class_stat st{};
         st.la = w.info->la; st.lb = w.info->lb; st.lc = w.info->lc; st.ld = w.info->ld;

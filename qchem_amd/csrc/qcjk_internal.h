@@ -1,12 +1,15 @@
 // qcjk_internal.h -- device-side data layout shared by the kernels and the C-ABI layer.
 //
 // HBM layout (all resident for the life of a handle; nothing O(N^4) is stored):
-//   shell pairs    : one global array, grouped by pair class (la>=lb: ss, ps, pp, ds, dp, dd), each
-//                    class sorted by descending Schwarz bound.  Per pair an int4 {bf offset of the
-//                    first shell, bf offset of the second, first primitive pair, nprim pairs | same
-//                    shell flag << 16} and a double4 {A-B (x,y,z), Q_raw}.
+//   molecules      : a handle holds one molecule or a batch; molecule m's matrices are an
+//                    n_m x n_m row-major block at element offset base_m of one packed array.
+//   shell pairs    : one global array, grouped by pair class (la>=lb: ss, ps, pp, ds, dp, dd),
+//                    within a class by molecule, each molecule's run sorted by descending Schwarz
+//                    bound.  Per pair an int4 {bf offset of the first shell in its molecule, the
+//                    same for the second, first primitive pair, nprim pairs | same shell flag << 16},
+//                    a double4 {A-B (x,y,z), Q_raw} and an int2 {base_m, n_m}.
 //   primitive pairs: two double4 per primitive pair: {P (x,y,z), p} and {P-A (x,y,z), c_i c_j K_AB}.
-//   density / Fock : D' = ns (x) ns o D (n x n), the unsymmetrised accumulators Jt, Kt (n x n).
+//   density / Fock : D' = ns (x) ns o D, the unsymmetrised accumulators Jt, Kt (packed blocks).
 //   Boys table     : 401 x 24 FP64 (77 KB).
 #pragma once
 
@@ -22,10 +25,10 @@ constexpr int pair_class(int la, int lb) { return la == 0 ? 0 : (la == 1 ? 1 + l
 enum : unsigned { kFlagJ = 1u, kFlagK = 2u, kFlagMixed = 4u, kFlagScreen = 8u };
 
 struct BuildArgs {
-    const double* D;       // D' (n x n), ns-prescaled
-    double* Jt;            // J accumulator (n x n), symmetrised later
+    const double* D;       // D', ns-prescaled, one n_m x n_m block per molecule (packed)
+    double* Jt;            // J accumulator (same layout), symmetrised later
     double* Kt;            // K accumulator
-    int n;
+    const int2* pmat;      // per shell pair: {element offset of its molecule's block, n_m}
     unsigned flags;
     const double* boys;    // kBoysRows x kBoysCols
     const double4* pp0;    // {Px,Py,Pz,p}
@@ -43,7 +46,7 @@ struct ClassPairArgs {
     int ket_off, nket;     // ket class range
     int same;              // bra class == ket class -> ket index <= bra index
     const long long* unit_start;  // [nbra+1] prefix sums of work units per bra pair
-    const int* nsurv;             // [nbra] surviving ket pairs (a prefix of the ket class list)
+    const int2* kseg;             // [nbra] {first ket pair of the bra's molecule, surviving ket pairs}
     long long nunits;
     int tile;              // ket pairs per unit (64 for the lane-per-quartet kernels, 1 otherwise)
 };

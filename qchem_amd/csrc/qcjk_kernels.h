@@ -23,6 +23,8 @@
 // (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.
 #pragma once
 
+#include <algorithm>
+
 #include "qcjk_launch.h"
 #include "mnd_quartet.h"
 #include "mnd_heavy2.h"
@@ -81,137 +83,142 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
     __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
     __shared__ HermiteAxis<LA, LB, float> sEab32[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
-    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (unit >= C.nunits) return;  // uniform per wavefront
-    const int P = find_unit_owner(C.unit_start, C.nbra, unit);
-    const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
-    bool active = q < C.nsurv[P];
-    const int Q = active ? q : 0;
-    const int4 pi = A.pinfo[C.bra_off + P];
-    const double4 pg = A.pgeo[C.bra_off + P];
-    const int4 qi = A.pinfo[C.ket_off + Q];
-    const double4 qg = A.pgeo[C.ket_off + Q];
-    if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
-    const bool fp32 = wave_fp32(A, active, pg, qg);
-    const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
+    for (long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); unit < C.nunits;
+         unit += (long long)gridDim.x * 4) {  // uniform per wavefront
+        const int P = find_unit_owner(C.unit_start, C.nbra, unit);
+        const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
+        const int2 seg = C.kseg[P];
+        bool active = q < seg.y;
+        const int Q = seg.x + (active ? q : 0);
+        const int2 mm = A.pmat[C.bra_off + P];
+        const int n = mm.y;
+        const double* __restrict__ D = A.D + mm.x;
+        double* __restrict__ Jt = A.Jt + mm.x;
+        double* __restrict__ Kt = A.Kt + mm.x;
+        const int4 pi = A.pinfo[C.bra_off + P];
+        const double4 pg = A.pgeo[C.bra_off + P];
+        const int4 qi = A.pinfo[C.ket_off + Q];
+        const double4 qg = A.pgeo[C.ket_off + Q];
+        if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
+        const bool fp32 = wave_fp32(A, active, pg, qg);
+        const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
 
-    double I[NAB * NCD];
-    static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = 0.0; });
-    if (active) {
-        if (fp32) {
-            float If[NAB * NCD];
-            static_for<0, NAB * NCD>([&](auto i) { If[decltype(i)::value] = 0.0f; });
-            light_compute<LA, LB, LC, LD, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
-            static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
-        } else {
-            light_compute<LA, LB, LC, LD, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
-        }
-    }
-
-    double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
-    if (!active) s = 0.0;
-    const int n = A.n;
-    const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
-    const double* __restrict__ D = A.D;
-
-    if (A.flags & kFlagJ) {
-        double dcd[NCD];
-        static_for<0, NCD>([&](auto i) {
-            constexpr int c = decltype(i)::value / ND, d = decltype(i)::value % ND;
-            dcd[decltype(i)::value] = active ? D[(oc + c) * n + od + d] : 0.0;
-        });
-        static_for<0, NAB>([&](auto ab) {
-            constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
-            double v = 0.0;
-            static_for<0, NCD>([&](auto cd) { v += I[decltype(ab)::value * NCD + decltype(cd)::value] * dcd[decltype(cd)::value]; });
-            v = wave_sum(2.0 * s * v);
-            if (lane == 0) unsafeAtomicAdd(&A.Jt[(oa + a) * n + ob + b], v);
-        });
-        double dab[NAB];
-        static_for<0, NAB>([&](auto i) {
-            constexpr int a = decltype(i)::value / NB, b = decltype(i)::value % NB;
-            dab[decltype(i)::value] = D[(oa + a) * n + ob + b];
-        });
+        double I[NAB * NCD];
+        static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = 0.0; });
         if (active) {
-            static_for<0, NCD>([&](auto cd) {
-                constexpr int c = decltype(cd)::value / ND, d = decltype(cd)::value % ND;
-                double v = 0.0;
-                static_for<0, NAB>([&](auto ab) { v += I[decltype(ab)::value * NCD + decltype(cd)::value] * dab[decltype(ab)::value]; });
-                unsafeAtomicAdd(&A.Jt[(oc + c) * n + od + d], 2.0 * s * v);
-            });
+            if (fp32) {
+                float If[NAB * NCD];
+                static_for<0, NAB * NCD>([&](auto i) { If[decltype(i)::value] = 0.0f; });
+                light_compute<LA, LB, LC, LD, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
+                static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
+            } else {
+                light_compute<LA, LB, LC, LD, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
+            }
         }
+
+        double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+        if (!active) s = 0.0;
+        const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
+
+        if (A.flags & kFlagJ) {
+            double dcd[NCD];
+            static_for<0, NCD>([&](auto i) {
+                constexpr int c = decltype(i)::value / ND, d = decltype(i)::value % ND;
+                dcd[decltype(i)::value] = active ? D[(oc + c) * n + od + d] : 0.0;
+            });
+            static_for<0, NAB>([&](auto ab) {
+                constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+                double v = 0.0;
+                static_for<0, NCD>([&](auto cd) { v += I[decltype(ab)::value * NCD + decltype(cd)::value] * dcd[decltype(cd)::value]; });
+                v = wave_sum(2.0 * s * v);
+                if (lane == 0) unsafeAtomicAdd(&Jt[(oa + a) * n + ob + b], v);
+            });
+            double dab[NAB];
+            static_for<0, NAB>([&](auto i) {
+                constexpr int a = decltype(i)::value / NB, b = decltype(i)::value % NB;
+                dab[decltype(i)::value] = D[(oa + a) * n + ob + b];
+            });
+            if (active) {
+                static_for<0, NCD>([&](auto cd) {
+                    constexpr int c = decltype(cd)::value / ND, d = decltype(cd)::value % ND;
+                    double v = 0.0;
+                    static_for<0, NAB>([&](auto ab) { v += I[decltype(ab)::value * NCD + decltype(cd)::value] * dab[decltype(ab)::value]; });
+                    unsafeAtomicAdd(&Jt[(oc + c) * n + od + d], 2.0 * s * v);
+                });
+            }
+        }
+        if ((A.flags & kFlagK) && active) {
+            // Kt_ac += s sum_bd I D_bd
+            {
+                double dd[NB * ND];
+                static_for<0, NB * ND>([&](auto i) {
+                    constexpr int b = decltype(i)::value / ND, d = decltype(i)::value % ND;
+                    dd[decltype(i)::value] = D[(ob + b) * n + od + d];
+                });
+                static_for<0, NA * NC>([&](auto x) {
+                    constexpr int a = decltype(x)::value / NC, c = decltype(x)::value % NC;
+                    double v = 0.0;
+                    static_for<0, NB * ND>([&](auto y) {
+                        constexpr int b = decltype(y)::value / ND, d = decltype(y)::value % ND;
+                        v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                    });
+                    unsafeAtomicAdd(&Kt[(oa + a) * n + oc + c], s * v);
+                });
+            }
+            // Kt_bc += s sum_ad I D_ad
+            {
+                double dd[NA * ND];
+                static_for<0, NA * ND>([&](auto i) {
+                    constexpr int a = decltype(i)::value / ND, d = decltype(i)::value % ND;
+                    dd[decltype(i)::value] = D[(oa + a) * n + od + d];
+                });
+                static_for<0, NB * NC>([&](auto x) {
+                    constexpr int b = decltype(x)::value / NC, c = decltype(x)::value % NC;
+                    double v = 0.0;
+                    static_for<0, NA * ND>([&](auto y) {
+                        constexpr int a = decltype(y)::value / ND, d = decltype(y)::value % ND;
+                        v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                    });
+                    unsafeAtomicAdd(&Kt[(ob + b) * n + oc + c], s * v);
+                });
+            }
+            // Kt_ad += s sum_bc I D_bc
+            {
+                double dd[NB * NC];
+                static_for<0, NB * NC>([&](auto i) {
+                    constexpr int b = decltype(i)::value / NC, c = decltype(i)::value % NC;
+                    dd[decltype(i)::value] = D[(ob + b) * n + oc + c];
+                });
+                static_for<0, NA * ND>([&](auto x) {
+                    constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
+                    double v = 0.0;
+                    static_for<0, NB * NC>([&](auto y) {
+                        constexpr int b = decltype(y)::value / NC, c = decltype(y)::value % NC;
+                        v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                    });
+                    unsafeAtomicAdd(&Kt[(oa + a) * n + od + d], s * v);
+                });
+            }
+            // Kt_bd += s sum_ac I D_ac
+            {
+                double dd[NA * NC];
+                static_for<0, NA * NC>([&](auto i) {
+                    constexpr int a = decltype(i)::value / NC, c = decltype(i)::value % NC;
+                    dd[decltype(i)::value] = D[(oa + a) * n + oc + c];
+                });
+                static_for<0, NB * ND>([&](auto x) {
+                    constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
+                    double v = 0.0;
+                    static_for<0, NA * NC>([&](auto y) {
+                        constexpr int a = decltype(y)::value / NC, c = decltype(y)::value % NC;
+                        v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
+                    });
+                    unsafeAtomicAdd(&Kt[(ob + b) * n + od + d], s * v);
+                });
+            }
+        }
+        if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
     }
-    if ((A.flags & kFlagK) && active) {
-        // Kt_ac += s sum_bd I D_bd
-        {
-            double dd[NB * ND];
-            static_for<0, NB * ND>([&](auto i) {
-                constexpr int b = decltype(i)::value / ND, d = decltype(i)::value % ND;
-                dd[decltype(i)::value] = D[(ob + b) * n + od + d];
-            });
-            static_for<0, NA * NC>([&](auto x) {
-                constexpr int a = decltype(x)::value / NC, c = decltype(x)::value % NC;
-                double v = 0.0;
-                static_for<0, NB * ND>([&](auto y) {
-                    constexpr int b = decltype(y)::value / ND, d = decltype(y)::value % ND;
-                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
-                });
-                unsafeAtomicAdd(&A.Kt[(oa + a) * n + oc + c], s * v);
-            });
-        }
-        // Kt_bc += s sum_ad I D_ad
-        {
-            double dd[NA * ND];
-            static_for<0, NA * ND>([&](auto i) {
-                constexpr int a = decltype(i)::value / ND, d = decltype(i)::value % ND;
-                dd[decltype(i)::value] = D[(oa + a) * n + od + d];
-            });
-            static_for<0, NB * NC>([&](auto x) {
-                constexpr int b = decltype(x)::value / NC, c = decltype(x)::value % NC;
-                double v = 0.0;
-                static_for<0, NA * ND>([&](auto y) {
-                    constexpr int a = decltype(y)::value / ND, d = decltype(y)::value % ND;
-                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
-                });
-                unsafeAtomicAdd(&A.Kt[(ob + b) * n + oc + c], s * v);
-            });
-        }
-        // Kt_ad += s sum_bc I D_bc
-        {
-            double dd[NB * NC];
-            static_for<0, NB * NC>([&](auto i) {
-                constexpr int b = decltype(i)::value / NC, c = decltype(i)::value % NC;
-                dd[decltype(i)::value] = D[(ob + b) * n + oc + c];
-            });
-            static_for<0, NA * ND>([&](auto x) {
-                constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
-                double v = 0.0;
-                static_for<0, NB * NC>([&](auto y) {
-                    constexpr int b = decltype(y)::value / NC, c = decltype(y)::value % NC;
-                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
-                });
-                unsafeAtomicAdd(&A.Kt[(oa + a) * n + od + d], s * v);
-            });
-        }
-        // Kt_bd += s sum_ac I D_ac
-        {
-            double dd[NA * NC];
-            static_for<0, NA * NC>([&](auto i) {
-                constexpr int a = decltype(i)::value / NC, c = decltype(i)::value % NC;
-                dd[decltype(i)::value] = D[(oa + a) * n + oc + c];
-            });
-            static_for<0, NB * ND>([&](auto x) {
-                constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
-                double v = 0.0;
-                static_for<0, NA * NC>([&](auto y) {
-                    constexpr int a = decltype(y)::value / NC, c = decltype(y)::value % NC;
-                    v += I[((a * NB + b) * NC + c) * ND + d] * dd[decltype(y)::value];
-                });
-                unsafeAtomicAdd(&A.Kt[(ob + b) * n + od + d], s * v);
-            });
-        }
-    }
-    if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -227,125 +234,130 @@ __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
     __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
     __shared__ HermiteAxis<LA, LB, float> sEab32[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
-    const long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (unit >= C.nunits) return;  // uniform per wavefront
-    const int P = find_unit_owner(C.unit_start, C.nbra, unit);
-    const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
-    bool active = q < C.nsurv[P];
-    const int Q = active ? q : 0;
-    const int4 pi = A.pinfo[C.bra_off + P];
-    const double4 pg = A.pgeo[C.bra_off + P];
-    const int4 qi = A.pinfo[C.ket_off + Q];
-    const double4 qg = A.pgeo[C.ket_off + Q];
-    if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
-    const bool fp32 = wave_fp32(A, active, pg, qg);
-    const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
-    double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
-    if (!active) s = 0.0;
-    const int n = A.n;
-    const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
-    const double* __restrict__ D = A.D;
-    const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
-    double jab[NAB], kad[NA * ND], kbd[NB * ND];
-    static_for<0, NAB>([&](auto i) { jab[decltype(i)::value] = 0.0; });
-    static_for<0, NA * ND>([&](auto i) { kad[decltype(i)::value] = 0.0; });
-    static_for<0, NB * ND>([&](auto i) { kbd[decltype(i)::value] = 0.0; });
-    static_for<0, NC>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        double I[NAB * ND];
-        static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = 0.0; });
-        if (active) {
-            if (fp32) {
-                float If[NAB * ND];
-                static_for<0, NAB * ND>([&](auto i) { If[decltype(i)::value] = 0.0f; });
-                light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
-                static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
-            } else {
-                light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
+    for (long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); unit < C.nunits;
+         unit += (long long)gridDim.x * 4) {  // uniform per wavefront
+        const int P = find_unit_owner(C.unit_start, C.nbra, unit);
+        const int q = (int)((unit - C.unit_start[P]) * 64) + lane;
+        const int2 seg = C.kseg[P];
+        bool active = q < seg.y;
+        const int Q = seg.x + (active ? q : 0);
+        const int2 mm = A.pmat[C.bra_off + P];
+        const int n = mm.y;
+        const double* __restrict__ D = A.D + mm.x;
+        double* __restrict__ Jt = A.Jt + mm.x;
+        double* __restrict__ Kt = A.Kt + mm.x;
+        const int4 pi = A.pinfo[C.bra_off + P];
+        const double4 pg = A.pgeo[C.bra_off + P];
+        const int4 qi = A.pinfo[C.ket_off + Q];
+        const double4 qg = A.pgeo[C.ket_off + Q];
+        if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
+        const bool fp32 = wave_fp32(A, active, pg, qg);
+        const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
+        double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
+        if (!active) s = 0.0;
+        const int oa = pi.x, ob = pi.y, oc = qi.x, od = qi.y;
+        const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+        double jab[NAB], kad[NA * ND], kbd[NB * ND];
+        static_for<0, NAB>([&](auto i) { jab[decltype(i)::value] = 0.0; });
+        static_for<0, NA * ND>([&](auto i) { kad[decltype(i)::value] = 0.0; });
+        static_for<0, NB * ND>([&](auto i) { kbd[decltype(i)::value] = 0.0; });
+        static_for<0, NC>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            double I[NAB * ND];
+            static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = 0.0; });
+            if (active) {
+                if (fp32) {
+                    float If[NAB * ND];
+                    static_for<0, NAB * ND>([&](auto i) { If[decltype(i)::value] = 0.0f; });
+                    light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
+                    static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
+                } else {
+                    light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
+                }
             }
-        }
-        if (active && doJ) {
-            static_for<0, ND>([&](auto dd) {
-                constexpr int d = decltype(dd)::value;
-                const double dcd = D[(oc + c) * n + od + d];
-                double v = 0.0;
-                static_for<0, NAB>([&](auto ab) {
-                    constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
-                    jab[decltype(ab)::value] += I[decltype(ab)::value * ND + d] * dcd;
-                    v += I[decltype(ab)::value * ND + d] * D[(oa + a) * n + ob + b];
-                });
-                unsafeAtomicAdd(&A.Jt[(oc + c) * n + od + d], 2.0 * s * v);
-            });
-        }
-        if (active && doK) {
-            static_for<0, NA>([&](auto aa) {  // Kt_ac += s sum_bd I D_bd
-                constexpr int a = decltype(aa)::value;
-                double v = 0.0;
-                static_for<0, NB>([&](auto bb) {
-                    constexpr int b = decltype(bb)::value;
-                    static_for<0, ND>([&](auto dd) {
-                        constexpr int d = decltype(dd)::value;
-                        v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + od + d];
-                    });
-                });
-                unsafeAtomicAdd(&A.Kt[(oa + a) * n + oc + c], s * v);
-            });
-            static_for<0, NB>([&](auto bb) {  // Kt_bc += s sum_ad I D_ad
-                constexpr int b = decltype(bb)::value;
-                double v = 0.0;
-                static_for<0, NA>([&](auto aa) {
-                    constexpr int a = decltype(aa)::value;
-                    static_for<0, ND>([&](auto dd) {
-                        constexpr int d = decltype(dd)::value;
-                        v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + od + d];
-                    });
-                });
-                unsafeAtomicAdd(&A.Kt[(ob + b) * n + oc + c], s * v);
-            });
-            static_for<0, NA>([&](auto aa) {  // Kt_ad += s sum_bc I D_bc (this chunk's c)
-                constexpr int a = decltype(aa)::value;
+            if (active && doJ) {
                 static_for<0, ND>([&](auto dd) {
                     constexpr int d = decltype(dd)::value;
+                    const double dcd = D[(oc + c) * n + od + d];
+                    double v = 0.0;
+                    static_for<0, NAB>([&](auto ab) {
+                        constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+                        jab[decltype(ab)::value] += I[decltype(ab)::value * ND + d] * dcd;
+                        v += I[decltype(ab)::value * ND + d] * D[(oa + a) * n + ob + b];
+                    });
+                    unsafeAtomicAdd(&Jt[(oc + c) * n + od + d], 2.0 * s * v);
+                });
+            }
+            if (active && doK) {
+                static_for<0, NA>([&](auto aa) {  // Kt_ac += s sum_bd I D_bd
+                    constexpr int a = decltype(aa)::value;
                     double v = 0.0;
                     static_for<0, NB>([&](auto bb) {
                         constexpr int b = decltype(bb)::value;
-                        v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + oc + c];
+                        static_for<0, ND>([&](auto dd) {
+                            constexpr int d = decltype(dd)::value;
+                            v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + od + d];
+                        });
                     });
-                    kad[a * ND + d] += v;
+                    unsafeAtomicAdd(&Kt[(oa + a) * n + oc + c], s * v);
                 });
-            });
-            static_for<0, NB>([&](auto bb) {  // Kt_bd += s sum_ac I D_ac (this chunk's c)
-                constexpr int b = decltype(bb)::value;
-                static_for<0, ND>([&](auto dd) {
-                    constexpr int d = decltype(dd)::value;
+                static_for<0, NB>([&](auto bb) {  // Kt_bc += s sum_ad I D_ad
+                    constexpr int b = decltype(bb)::value;
                     double v = 0.0;
                     static_for<0, NA>([&](auto aa) {
                         constexpr int a = decltype(aa)::value;
-                        v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + oc + c];
+                        static_for<0, ND>([&](auto dd) {
+                            constexpr int d = decltype(dd)::value;
+                            v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + od + d];
+                        });
                     });
-                    kbd[b * ND + d] += v;
+                    unsafeAtomicAdd(&Kt[(ob + b) * n + oc + c], s * v);
                 });
+                static_for<0, NA>([&](auto aa) {  // Kt_ad += s sum_bc I D_bc (this chunk's c)
+                    constexpr int a = decltype(aa)::value;
+                    static_for<0, ND>([&](auto dd) {
+                        constexpr int d = decltype(dd)::value;
+                        double v = 0.0;
+                        static_for<0, NB>([&](auto bb) {
+                            constexpr int b = decltype(bb)::value;
+                            v += I[(a * NB + b) * ND + d] * D[(ob + b) * n + oc + c];
+                        });
+                        kad[a * ND + d] += v;
+                    });
+                });
+                static_for<0, NB>([&](auto bb) {  // Kt_bd += s sum_ac I D_ac (this chunk's c)
+                    constexpr int b = decltype(bb)::value;
+                    static_for<0, ND>([&](auto dd) {
+                        constexpr int d = decltype(dd)::value;
+                        double v = 0.0;
+                        static_for<0, NA>([&](auto aa) {
+                            constexpr int a = decltype(aa)::value;
+                            v += I[(a * NB + b) * ND + d] * D[(oa + a) * n + oc + c];
+                        });
+                        kbd[b * ND + d] += v;
+                    });
+                });
+            }
+        });
+        if (doJ) {
+            static_for<0, NAB>([&](auto ab) {
+                constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
+                const double v = wave_sum(2.0 * s * jab[decltype(ab)::value]);
+                if (lane == 0) unsafeAtomicAdd(&Jt[(oa + a) * n + ob + b], v);
             });
         }
-    });
-    if (doJ) {
-        static_for<0, NAB>([&](auto ab) {
-            constexpr int a = decltype(ab)::value / NB, b = decltype(ab)::value % NB;
-            const double v = wave_sum(2.0 * s * jab[decltype(ab)::value]);
-            if (lane == 0) unsafeAtomicAdd(&A.Jt[(oa + a) * n + ob + b], v);
-        });
+        if (doK && active) {
+            static_for<0, NA * ND>([&](auto x) {
+                constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
+                unsafeAtomicAdd(&Kt[(oa + a) * n + od + d], s * kad[decltype(x)::value]);
+            });
+            static_for<0, NB * ND>([&](auto x) {
+                constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
+                unsafeAtomicAdd(&Kt[(ob + b) * n + od + d], s * kbd[decltype(x)::value]);
+            });
+        }
+        if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
     }
-    if (doK && active) {
-        static_for<0, NA * ND>([&](auto x) {
-            constexpr int a = decltype(x)::value / ND, d = decltype(x)::value % ND;
-            unsafeAtomicAdd(&A.Kt[(oa + a) * n + od + d], s * kad[decltype(x)::value]);
-        });
-        static_for<0, NB * ND>([&](auto x) {
-            constexpr int b = decltype(x)::value / ND, d = decltype(x)::value % ND;
-            unsafeAtomicAdd(&A.Kt[(ob + b) * n + od + d], s * kbd[decltype(x)::value]);
-        });
-    }
-    if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -364,7 +376,8 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
     for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
         const int P = find_unit_owner(C.unit_start, C.nbra, u);
-        const int Q = (int)(u - C.unit_start[P]);
+        const int Q = C.kseg[P].x + (int)(u - C.unit_start[P]);
+        const int2 mm = A.pmat[C.bra_off + P];
         const int4 pi = A.pinfo[C.bra_off + P];
         const double4 pg = A.pgeo[C.bra_off + P];
         const int4 qi = A.pinfo[C.ket_off + Q];
@@ -375,9 +388,9 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
         for (int t = lane; t < Dg::NTASK; t += 64) {
             int row, col;
             double val;
-            if (Dg::task(t, sm.XI, A.D, A.n, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
-                double* tgt = t < Dg::T2 ? A.Jt : A.Kt;
-                unsafeAtomicAdd(&tgt[row * A.n + col], val);
+            if (Dg::task(t, sm.XI, A.D + mm.x, mm.y, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
+                double* tgt = (t < Dg::T2 ? A.Jt : A.Kt) + mm.x;
+                unsafeAtomicAdd(&tgt[row * mm.y + col], val);
             }
         }
         if (A.counters && lane == 0) {
@@ -405,7 +418,8 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
     for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
         const int P = find_unit_owner(C.unit_start, C.nbra, u);
-        const int Q = (int)(u - C.unit_start[P]);
+        const int Q = C.kseg[P].x + (int)(u - C.unit_start[P]);
+        const int2 mm = A.pmat[C.bra_off + P];
         const int4 pi = A.pinfo[C.bra_off + P];
         const double4 pg = A.pgeo[C.bra_off + P];
         const int4 qi = A.pinfo[C.ket_off + Q];
@@ -418,9 +432,9 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
         for (int t = lane; t < Dg::NTASK; t += 64) {
             int row, col;
             double val;
-            if (Dg::task(t, sI, A.D, A.n, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
-                double* tgt = t < Dg::T2 ? A.Jt : A.Kt;
-                unsafeAtomicAdd(&tgt[row * A.n + col], val);
+            if (Dg::task(t, sI, A.D + mm.x, mm.y, pi.x, pi.y, qi.x, qi.y, doJ, doK, s, row, col, val)) {
+                double* tgt = (t < Dg::T2 ? A.Jt : A.Kt) + mm.x;
+                unsafeAtomicAdd(&tgt[row * mm.y + col], val);
             }
         }
         if (A.counters && lane == 0) {
@@ -435,11 +449,15 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Launchers + dispatch table
 // ---------------------------------------------------------------------------------------------
+// Lane kernels loop over their work units with a grid stride: one launch per class covers every
+// molecule of a batch, and the grid is capped at 64 four-wave blocks per CU (256 CUs).
+constexpr long long kLaneGridCap = 256LL * 64;
+
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_light(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
-    const long long blocks = (c.nunits + 3) / 4;
+    const long long blocks = std::min<long long>((c.nunits + 3) / 4, kLaneGridCap);
     hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
     return hipGetLastError();
 }
@@ -448,7 +466,7 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_lightc(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
-    const long long blocks = (c.nunits + 3) / 4;
+    const long long blocks = std::min<long long>((c.nunits + 3) / 4, kLaneGridCap);
     hipLaunchKernelGGL((k_jk_lightc<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
     return hipGetLastError();
 }

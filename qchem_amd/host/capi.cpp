@@ -2,7 +2,9 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "qchem_host.h"
 #include "scf.hpp"
@@ -99,6 +101,88 @@ int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const dou
         for (size_t i = 0; i < r.energy_history.size() && (int)i < par->max_iter; ++i) {
             if (e_hist) e_hist[i] = r.energy_history[i];
             if (t_hist) t_hist[i] = r.time_history[i];
+        }
+        return 0;
+    } catch (const EngineError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 100;
+    }
+}
+
+int qcscf_rhf_batch(const qchost_shells* mols, int32_t nmol, const int32_t* natom, const double* Z, const double* R,
+                    const int32_t* charge, const qcscf_params* par, int32_t nthreads, qcscf_result* res,
+                    double* e_hist, qcscf_batch_result* bres)
+{
+    try {
+        if (!mols || nmol < 1 || !natom || !Z || !R || !par || !res) throw std::runtime_error("qcscf_rhf_batch: null argument");
+        std::vector<Basis> bases;
+        bases.reserve(nmol);
+        std::vector<const Basis*> bp;
+        std::vector<std::vector<Atom>> atoms(nmol);
+        std::vector<int> nelec(nmol);
+        int a0 = 0;
+        for (int m = 0; m < nmol; ++m) {
+            bases.push_back(make_basis(&mols[m]));
+            double zsum = 0.0;
+            atoms[m].resize(natom[m]);
+            for (int i = 0; i < natom[m]; ++i) {
+                atoms[m][i].Z = Z[a0 + i];
+                zsum += Z[a0 + i];
+                for (int k = 0; k < 3; ++k) atoms[m][i].R[k] = R[3 * (a0 + i) + k];
+            }
+            a0 += natom[m];
+            nelec[m] = (int)std::lround(zsum) - (charge ? charge[m] : 0);
+        }
+        for (const Basis& b : bases) bp.push_back(&b);
+        qcjk_opts o{};
+        o.device = par->device;
+        o.screen_tol = par->screen_tol;
+        o.mixed_tol = par->mixed_tol;
+        SCFParams p;
+        p.max_iter = par->max_iter;
+        p.e_tol = par->e_tol;
+        p.fd_tol = par->fd_tol;
+        p.nproj = par->nproj;
+        p.fd_max = par->fd_max;
+        p.sv_tol = par->sv_tol;
+        p.verbose = false;
+        const BatchSCFResult r = rhf_batch(bp, atoms, nelec, p, o, par->jk_flags ? par->jk_flags : (QCJK_J | QCJK_K),
+                                           nthreads);
+        int nconv = 0;
+        for (int m = 0; m < nmol; ++m) {
+            const SCFResult& x = r.mols[m];
+            qcscf_result& y = res[m];
+            std::memset(&y, 0, sizeof(y));
+            y.energy = x.energy;
+            y.e_nuc = x.e_nuc;
+            y.e_one = x.e_one;
+            y.e_coul = x.e_coul;
+            y.e_exch = x.e_exch;
+            y.iterations = x.iterations;
+            y.converged = x.converged ? 1 : 0;
+            y.nbf = bases[m].nbf;
+            y.t_total = r.t_total;
+            y.t_jk = r.t_jk;
+            y.t_diag = x.t_diag;
+            y.t_setup = r.t_setup;
+            nconv += x.converged ? 1 : 0;
+            if (e_hist)
+                for (size_t i = 0; i < x.energy_history.size() && (int)i < par->max_iter; ++i)
+                    e_hist[(size_t)m * par->max_iter + i] = x.energy_history[i];
+        }
+        if (bres) {
+            bres->iterations = r.iterations;
+            bres->nconverged = nconv;
+            bres->t_total = r.t_total;
+            bres->t_setup = r.t_setup;
+            bres->t_jk = r.t_jk;
+            bres->t_host = r.t_host;
+            bres->jk_device_ms = r.jk_device_ms;
+            bres->quartets = r.quartets;
+            bres->quartets_fp32 = r.quartets_fp32;
         }
         return 0;
     } catch (const EngineError& e) {

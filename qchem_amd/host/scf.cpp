@@ -1,11 +1,15 @@
 // scf.cpp -- see scf.hpp.
 #include "scf.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <deque>
+#include <exception>
 #include <stdexcept>
+#include <thread>
 
 namespace qchem_amd {
 
@@ -103,6 +107,27 @@ struct DIIS {
     }
 };
 
+// Linear density mixing with the adaptive relaxation of DensityMixer.C:104-139, as the
+// fixed-point loop drives it (SCFIterator.C:265-351; StartingRelaxRo = 1, SCFParams.C:32):
+// D_in <- r D_out + (1 - r) D_in with r = relax, or r = relax/4 and relax *= 0.8 when ||[F,D]||
+// rose since the previous iteration; relax *= 1.5 when it fell; relax <= RelMax, which drops from 1
+// to 0.5 once ||D_out - D_in||_F / N < 1e-5.
+struct LinearMixer {
+    double relax = 1.0, relmax = 1.0;
+    Mat step(const Mat& Din, const Mat& Dout, double fd, double fd_old, double nelec)
+    {
+        if (frob(axpby(1.0, Dout, -1.0, Din)) / nelec < 1e-5) relmax = 0.5;
+        double r = relax;
+        if (fd > fd_old && std::fabs(fd - fd_old) > 1e-9) {
+            r = relax / 4.0;
+            relax *= 0.8;
+        }
+        if (fd < fd_old) relax *= 1.5;
+        relax = std::min(relax, relmax);
+        return axpby(r, Dout, 1.0 - r, Din);
+    }
+};
+
 Mat density_from(const Ortho& X, const Mat& Fp, int nocc, std::vector<double>& eps, double& t_diag)
 {
     const auto t0 = clk::now();
@@ -154,50 +179,49 @@ void one_electron(GpuHF_IBS& eng, const std::vector<Atom>& atoms, Mat& S, Mat& h
     h = axpby(1.0, T, 1.0, V);
 }
 
-SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0)
-{
-    SCFResult res;
-    const auto t_start = clk::now();
-    if (nelec <= 0 || nelec % 2) throw std::runtime_error("rhf: closed-shell electron count required");
-    const int nocc = nelec / 2;
-    const int n = eng.nbf();
-    Mat S, h;
-    one_electron(eng, atoms, S, h);
+// One closed-shell RHF run after its set-up: the part of the loop that follows each Fock build.
+// Shared by rhf() (one molecule through the HF faces) and rhf_batch() (many molecules, one batched
+// build per iteration), so both run the same recipe.
+struct RHFRun {
+    const SCFParams* par = nullptr;
+    int n = 0, nocc = 0;
+    Mat h;
     Ortho X;
-    X.set(S);
-    if (nocc > X.m) throw std::runtime_error("rhf: more occupied orbitals than basis functions");
-    res.e_nuc = nuclear_repulsion(atoms);
-    double Ztot = 0.0;
-    for (const Atom& a : atoms) Ztot += a.Z;
-    DIIS diis{par.nproj, par.fd_max > 0 ? par.fd_max : (double)((int)Ztot * (int)Ztot) * 0.1 / 32, par.sv_tol, {}, {}};
+    DIIS diis{4, 0.0, 0.0, {}, {}};
+    Mat D;
+    double E_old = 0.0, fd_old = 1e10;
+    LinearMixer mixer;
+    bool done = false;
+    SCFResult res;
 
-    // The Fock operator terms as the reference's Hamiltonian holds them: the two HF terms reach the
-    // basis through the WholeSystemFock face (J first, then K, same density -> one fused build).
-    const WholeSystemFock_IBS<double>& whole = eng;
-    Mat D = D0 ? *D0 : density_from(X, X.to_ortho(h), nocc, res.eps, res.t_diag);
-    if (D.n != n) throw std::runtime_error("rhf: seed density has the wrong dimension");
-    res.t_setup = secs(t_start, clk::now());
+    void setup(const Mat& S, const Mat& h1, const std::vector<Atom>& atoms, int nelec, const SCFParams& p,
+               const Mat* D0)
+    {
+        par = &p;
+        if (nelec <= 0 || nelec % 2) throw std::runtime_error("rhf: closed-shell electron count required");
+        nocc = nelec / 2;
+        n = S.n;
+        h = h1;
+        X.set(S);
+        if (nocc > X.m) throw std::runtime_error("rhf: more occupied orbitals than basis functions");
+        res.e_nuc = nuclear_repulsion(atoms);
+        double Ztot = 0.0;
+        for (const Atom& a : atoms) Ztot += a.Z;
+        diis = DIIS{p.nproj, p.fd_max > 0 ? p.fd_max : (double)((int)Ztot * (int)Ztot) * 0.1 / 32, p.sv_tol, {}, {}};
+        D = D0 ? *D0 : density_from(X, X.to_ortho(h), nocc, res.eps, res.t_diag);
+        if (D.n != n) throw std::runtime_error("rhf: seed density has the wrong dimension");
+    }
 
-    double E_old = 0.0;
-    for (int it = 1; it <= par.max_iter; ++it) {
-        const auto tj = clk::now();
-        Mat Dao(n);
-        whole.AddAODensity(Dao, D);
-        const Mat J = whole.MakeAOFock(Dao, false);
-        const Mat K = whole.MakeAOFock(Dao, true);
-        res.t_jk += secs(tj, clk::now());
-        Mat F = h;
-        whole.SliceAOFock(F, J);
-        Mat mhK = K;
-        mhK *= -0.5;
-        whole.SliceAOFock(F, mhK);
+    // F = h + J - 1/2 K was built from D.  Energy, convergence test, DIIS, new D.  Returns done.
+    bool after_build(int it, const Mat& F, const Mat& J, const Mat& K, double t_elapsed)
+    {
         res.e_one = dot(D, h);
         res.e_coul = 0.5 * dot(D, J);
         res.e_exch = -0.25 * dot(D, K);
         const double E = res.e_one + res.e_coul + res.e_exch + res.e_nuc;
         res.energy = E;
         res.energy_history.push_back(E);
-        res.time_history.push_back(secs(t_start, clk::now()));
+        res.time_history.push_back(t_elapsed);
         res.iterations = it;
 
         const Mat Fp = X.to_ortho(F);
@@ -205,22 +229,155 @@ SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const S
         const Mat Ep = axpby(1.0, Fp * Dp, -1.0, Dp * Fp);
         const double fd = frob(Ep);
         const double dE = it > 1 ? std::fabs(E - E_old) : 1e300;
-        if (par.verbose)
+        if (par->verbose)
             std::fprintf(stderr, "  it %3d  E = %.12f  dE = %.3e  [F,D] = %.3e\n", it, E, it > 1 ? E - E_old : 0.0, fd);
-        if (dE < par.e_tol && fd < par.fd_tol) {
+        if (dE < par->e_tol && fd < par->fd_tol) {
             res.converged = true;
             res.D = D;
-            break;
+            done = true;
+            return true;
         }
         E_old = E;
         const Mat Fx = diis.next(Fp, Ep, fd);
-        D = density_from(X, Fx, nocc, res.eps, res.t_diag);
+        D = mixer.step(D, density_from(X, Fx, nocc, res.eps, res.t_diag), fd, fd_old, 2.0 * nocc);
+        fd_old = fd;
         res.D = D;
+        return false;
     }
+};
+
+SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0)
+{
+    const auto t_start = clk::now();
+    const int n = eng.nbf();
+    Mat S, h;
+    one_electron(eng, atoms, S, h);
+    RHFRun run;
+    run.setup(S, h, atoms, nelec, par, D0);
+    run.res.t_setup = secs(t_start, clk::now());
+
+    // The Fock operator terms as the reference's Hamiltonian holds them: the two HF terms reach the
+    // basis through the WholeSystemFock face (J first, then K, same density -> one fused build).
+    const WholeSystemFock_IBS<double>& whole = eng;
+    for (int it = 1; it <= par.max_iter; ++it) {
+        const auto tj = clk::now();
+        Mat Dao(n);
+        whole.AddAODensity(Dao, run.D);
+        const Mat J = whole.MakeAOFock(Dao, false);
+        const Mat K = whole.MakeAOFock(Dao, true);
+        run.res.t_jk += secs(tj, clk::now());
+        Mat F = run.h;
+        whole.SliceAOFock(F, J);
+        Mat mhK = K;
+        mhK *= -0.5;
+        whole.SliceAOFock(F, mhK);
+        if (run.after_build(it, F, J, K, secs(t_start, clk::now()))) break;
+    }
+    SCFResult res = std::move(run.res);
     res.t_total = secs(t_start, clk::now());
     res.quartets = eng.quartets_total();
     res.quartets_fp32 = eng.quartets_fp32_total();
     return res;
+}
+
+BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vector<std::vector<Atom>>& atoms,
+                         const std::vector<int>& nelec, const SCFParams& par, const qcjk_opts& opts, unsigned jk_flags,
+                         int nthreads)
+{
+    const auto t_start = clk::now();
+    const int nmol = (int)bases.size();
+    if (nmol == 0 || (int)atoms.size() != nmol || (int)nelec.size() != nmol)
+        throw std::runtime_error("rhf_batch: bases, atoms and electron counts differ in length");
+    BatchSCFResult out;
+    std::vector<qcjk_basis> views(nmol);
+    for (int m = 0; m < nmol; ++m) views[m] = bases[m]->view();
+    qcjk_handle* h = nullptr;
+    int rc = qcjk_create_batch(views.data(), nmol, &opts, &h);
+    if (rc != QCJK_OK) throw EngineError("qcjk_create_batch", rc);
+    struct Guard {
+        qcjk_handle* h;
+        ~Guard() { qcjk_destroy(h); }
+    } guard{h};
+    std::vector<int32_t> nbf(nmol);
+    std::vector<int64_t> off(nmol);
+    int64_t nelem = 0;
+    rc = qcjk_layout(h, nullptr, nbf.data(), off.data(), &nelem);
+    if (rc != QCJK_OK) throw EngineError("qcjk_layout", rc);
+
+    // per-molecule work on a small thread pool (molecules are independent)
+    nthreads = std::max(1, std::min(nthreads, nmol));
+    auto parallel = [&](auto&& fn) {
+        std::atomic<int> next{0};
+        std::vector<std::exception_ptr> errs(nthreads);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; ++t)
+            pool.emplace_back([&, t] {
+                try {
+                    for (int m = next++; m < nmol; m = next++) fn(m);
+                } catch (...) {
+                    errs[t] = std::current_exception();
+                }
+            });
+        for (auto& th : pool) th.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
+    };
+
+    std::vector<RHFRun> runs(nmol);
+    std::vector<Mat> S(nmol), h1(nmol);
+    for (int m = 0; m < nmol; ++m) {  // one-electron integrals on the device, one molecule at a time
+        const int n = nbf[m];
+        std::vector<double> Z(atoms[m].size()), R(3 * atoms[m].size());
+        for (size_t i = 0; i < atoms[m].size(); ++i) {
+            Z[i] = atoms[m][i].Z;
+            for (int x = 0; x < 3; ++x) R[3 * i + x] = atoms[m][i].R[x];
+        }
+        S[m] = Mat(n);
+        Mat T(n), V(n);
+        rc = qcjk_one_electron_mol(h, m, (int)Z.size(), Z.data(), R.data(), S[m].a.data(), T.a.data(), V.a.data());
+        if (rc != QCJK_OK) throw EngineError("qcjk_one_electron_mol", rc);
+        h1[m] = axpby(1.0, T, 1.0, V);
+    }
+    parallel([&](int m) { runs[m].setup(S[m], h1[m], atoms[m], nelec[m], par, nullptr); });
+    out.t_setup = secs(t_start, clk::now());
+
+    std::vector<double> Dp(nelem), Jp(nelem), Kp(nelem);
+    for (int it = 1; it <= par.max_iter; ++it) {
+        const auto tj = clk::now();
+        for (int m = 0; m < nmol; ++m) std::memcpy(Dp.data() + off[m], runs[m].D.a.data(), runs[m].D.a.size() * sizeof(double));
+        qcjk_stats st{};
+        rc = qcjk_build(h, Dp.data(), Jp.data(), Kp.data(), jk_flags | QCJK_GRAPH, &st);
+        if (rc != QCJK_OK) throw EngineError("qcjk_build", rc);
+        out.t_jk += secs(tj, clk::now());
+        out.jk_device_ms += st.device_ms;
+        out.quartets += st.quartets_computed;
+        out.quartets_fp32 += st.quartets_fp32;
+        out.iterations = it;
+        const auto th = clk::now();
+        const double t_now = secs(t_start, th);
+        parallel([&](int m) {
+            RHFRun& r = runs[m];
+            if (r.done) return;
+            const int n = nbf[m];
+            Mat J(n), K(n);
+            std::memcpy(J.a.data(), Jp.data() + off[m], J.a.size() * sizeof(double));
+            std::memcpy(K.a.data(), Kp.data() + off[m], K.a.size() * sizeof(double));
+            Mat F = r.h;
+            F += J;
+            Mat mhK = K;
+            mhK *= -0.5;
+            F += mhK;
+            r.after_build(it, F, J, K, t_now);
+        });
+        out.t_host += secs(th, clk::now());
+        bool all = true;
+        for (const RHFRun& r : runs) all = all && r.done;
+        if (all) break;
+    }
+    out.t_total = secs(t_start, clk::now());
+    out.mols.reserve(nmol);
+    for (RHFRun& r : runs) out.mols.push_back(std::move(r.res));
+    return out;
 }
 
 }  // namespace qchem_amd

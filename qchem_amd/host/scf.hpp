@@ -10,6 +10,8 @@
 //                                                                    Math/DIIS.C:24-51, Calculation.C:184
 //   loop          Fock from D, accelerator, diagonalise, aufbau fill, D = 2 sum_occ C C^T
 //                                                                    SCFIterator.C:240-362, LoopDriver.C:56-70
+//   mixing        D_in <- r D_out + (1-r) D_in, adaptive r (re-damp to r/4 when [F,D] rises)
+//                                                                    DensityMixer.C:104-139
 #pragma once
 
 #include <vector>
@@ -54,5 +56,20 @@ void one_electron(GpuHF_IBS& eng, const std::vector<Atom>& atoms, Mat& S, Mat& h
 
 // RHF to convergence.  D0 (optional) is an explicit seed density; otherwise the core guess.
 SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0 = nullptr);
+
+// Many independent molecules, each running the rhf() recipe, with ONE batched J/K build per
+// iteration for all of them (qcjk_create_batch; the build replays a captured hipGraph).  A
+// molecule that has converged keeps its density; the loop ends when all have converged.
+// Per-molecule host work (Fock assembly, DIIS, diagonalisation) runs on nthreads threads.
+struct BatchSCFResult {
+    std::vector<SCFResult> mols;
+    int iterations = 0;                  // batched builds
+    double t_total = 0, t_setup = 0, t_jk = 0, t_host = 0;
+    double jk_device_ms = 0;             // device time of the builds
+    long long quartets = 0, quartets_fp32 = 0;
+};
+BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vector<std::vector<Atom>>& atoms,
+                         const std::vector<int>& nelec, const SCFParams& par, const qcjk_opts& opts, unsigned jk_flags,
+                         int nthreads);
 
 }  // namespace qchem_amd

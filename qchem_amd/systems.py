@@ -150,8 +150,8 @@ def alkane_chain(ncarbon=16, seed=None, jitter=0.05):
             xyz.append([c[k] + dch * d[k] for k in range(3)])
     xyz = [[x + jitter * rng.normal() for x in p] for p in xyz]
     shells = [s for Z, c in zip(Zs, xyz) for s in even_tempered_shells(Z, c)]
-    return dict(name="C%dH%d zig-zag chain (%d functions)" % (ncarbon, 2 * ncarbon, 0), Z=Zs, R=xyz, shells=shells,
-                charge=0)
+    return dict(name="C%dH%d zig-zag chain (%d functions)" % (ncarbon, 2 * ncarbon, nbf(shells)), Z=Zs, R=xyz,
+                shells=shells, charge=0)
 
 
 def batch(nmol=512, seed=None, lo=2, hi=12):
