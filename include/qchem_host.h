@@ -39,6 +39,7 @@ typedef struct qcscf_params {
     double mixed_tol;
     int32_t device;
     int32_t verbose;
+    int32_t guess;     /* seed when no D0 is given: 0 = core Hamiltonian, 1 = SAD (atomic densities) */
 } qcscf_params;
 
 typedef struct qcscf_result {
@@ -60,11 +61,12 @@ int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const dou
               const qcscf_params* par, const double* D0, qcscf_result* res, double* D_out, double* e_hist,
               double* t_hist);
 
-/* Batched closed-shell RHF: nmol independent molecules, each running the qcscf_rhf recipe (core
- * guess), with ONE batched J/K build per iteration for all of them (qcjk_create_batch, replayed
+/* Batched closed-shell RHF: nmol independent molecules, each running the qcscf_rhf recipe (seed:
+ * D0 if given -- the molecules' n_m x n_m seeds back to back -- else par->guess), with ONE batched J/K build per iteration for all of them (qcjk_create_batch, replayed
  * hipGraph).  natom[m] atoms per molecule; Z and R hold all molecules' atoms back to back; charge
  * is per molecule (nullable = neutral).  res[m] receives each molecule's result (its t_* fields
- * are the shared batch timings); e_hist (nullable, nmol x max_iter) the energy histories.
+ * are the shared batch timings); e_hist (nullable, nmol x max_iter) the energy histories; D_out
+ * (nullable) the final densities, back to back like D0.
  * Per-molecule host work runs on nthreads threads.  No reference counterpart. */
 typedef struct qcscf_batch_result {
     int32_t iterations;      /* batched builds */
@@ -75,8 +77,8 @@ typedef struct qcscf_batch_result {
 } qcscf_batch_result;
 
 int qcscf_rhf_batch(const qchost_shells* mols, int32_t nmol, const int32_t* natom, const double* Z, const double* R,
-                    const int32_t* charge, const qcscf_params* par, int32_t nthreads, qcscf_result* res,
-                    double* e_hist, qcscf_batch_result* bres);
+                    const int32_t* charge, const qcscf_params* par, const double* D0, int32_t nthreads,
+                    qcscf_result* res, double* e_hist, double* D_out, qcscf_batch_result* bres);
 
 /* Last error message of this thread (exception text), or "". */
 const char* qchost_last_error(void);

@@ -75,7 +75,9 @@ class _ScfParams(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("e_tol", ctypes.c_double), ("fd_tol", ctypes.c_double),
                 ("nproj", ctypes.c_int32), ("fd_max", ctypes.c_double), ("sv_tol", ctypes.c_double),
                 ("jk_flags", ctypes.c_uint32), ("screen_tol", ctypes.c_double), ("mixed_tol", ctypes.c_double),
-                ("device", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("verbose", ctypes.c_int32), ("guess", ctypes.c_int32)]
+
+GUESS_CORE, GUESS_SAD = 0, 1
 
 
 class _ScfResult(ctypes.Structure):
@@ -166,9 +168,10 @@ def host_lib():
                                 ctypes.POINTER(ctypes.c_double)]
         H.qcscf_rhf_batch.argtypes = [ctypes.POINTER(_Shells), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_ScfParams), ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_ScfParams),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
                                       ctypes.POINTER(_ScfResult), ctypes.POINTER(ctypes.c_double),
-                                      ctypes.POINTER(_BatchResult)]
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_BatchResult)]
         H.qchost_last_error.restype = ctypes.c_char_p
         _HOST = H
     return _HOST
@@ -364,12 +367,12 @@ class JKEngine:
 
 
 def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_max=-1.0, sv_tol=5e-9,
-        jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, D0=None, verbose=False):
+        jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, D0=None, verbose=False, guess=GUESS_CORE):
     """Closed-shell RHF through the C++ driver; returns a dict with energy, density and history."""
     Z = np.ascontiguousarray(Z, dtype=np.float64)
     R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
     par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device,
-                     1 if verbose else 0)
+                     1 if verbose else 0, int(guess))
     res = _ScfResult()
     n = basis.nbf
     Dout = np.zeros((n, n))
@@ -391,7 +394,8 @@ def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, f
 
 
 def rhf_batch(bases, mols, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_max=-1.0, sv_tol=5e-9,
-              jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, nthreads=16):
+              jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, nthreads=16, guess=GUESS_CORE,
+              D0=None, verbose=False):
     """Batched closed-shell RHF (qcscf_rhf_batch): one batched J/K build per iteration for every
     molecule.  ``mols``: dicts with Z, R (and optional charge).  Returns (per-molecule list, summary)."""
     nmol = len(bases)
@@ -400,19 +404,30 @@ def rhf_batch(bases, mols, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_ma
     Z = np.ascontiguousarray(np.concatenate([np.asarray(m["Z"], dtype=np.float64) for m in mols]))
     R = np.ascontiguousarray(np.concatenate([np.asarray(m["R"], dtype=np.float64).reshape(-1) for m in mols]))
     charge = np.array([int(m.get("charge", 0)) for m in mols], dtype=np.int32)
-    par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device, 0)
+    par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device,
+                     1 if verbose else 0, int(guess))
+    d0 = None
+    if D0 is not None:  # per-molecule seeds, packed back to back
+        d0 = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in D0]))
     res = (_ScfResult * nmol)()
     eh = np.zeros(nmol * max_iter)
+    nbfs = [b.nbf for b in bases]
+    dout = np.zeros(sum(n * n for n in nbfs))
     br = _BatchResult()
     rc = host_lib().qcscf_rhf_batch(shells, nmol, _ip(natom), _dp(Z), _dp(R), _ip(charge), ctypes.byref(par),
-                                    int(nthreads), res, _dp(eh), ctypes.byref(br))
+                                    _dp(d0) if d0 is not None else None, int(nthreads), res, _dp(eh), _dp(dout),
+                                    ctypes.byref(br))
     if rc != 0:
         raise EngineError("qcscf_rhf_batch", rc, host_lib().qchost_last_error().decode())
     out = []
+    off = 0
     for m in range(nmol):
         r = res[m]
+        n = nbfs[m]
         out.append({"energy": r.energy, "iterations": r.iterations, "converged": bool(r.converged), "nbf": r.nbf,
-                    "energy_history": eh[m * max_iter:m * max_iter + r.iterations].copy()})
+                    "energy_history": eh[m * max_iter:m * max_iter + r.iterations].copy(),
+                    "D": dout[off:off + n * n].reshape(n, n).copy()})
+        off += n * n
     summary = {k: getattr(br, k) for k, _ in _BatchResult._fields_}
     return out, summary
 

@@ -31,6 +31,7 @@ HIP_FLAGS = [
 HIP_SOURCES = ["qcjk_k_light_c.hip", "qcjk_k_light_b.hip", "qcjk_k_lightc.hip", "qcjk_k_light_a.hip",
                "qcjk_k_heavy2_c.hip", "qcjk_k_heavy2_d.hip", "qcjk_k_heavy2_b.hip", "qcjk_k_heavy2_a.hip",
                "qcjk_k_heavy.hip", "qcjk_eri.hip", "qcjk_setup.hip", "qcjk_api.hip"]
+ML_SOURCE = "qcml.hip"
 HOST_SOURCES = ["linalg.cpp", "scf.cpp", "capi.cpp"]
 
 
@@ -56,6 +57,13 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", lib + ".tmp"] + hip_objs)
     os.replace(lib + ".tmp", lib)
 
+    # learned-guess network: its own library (include/qcml.h)
+    mobj = os.path.join(OBJ, "qcml.o")
+    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, ML_SOURCE), "-o", mobj])
+    mlib = os.path.join(HERE, "libqcml.so")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", mlib + ".tmp", mobj])
+    os.replace(mlib + ".tmp", mlib)
+
     cxx = shutil.which("g++") or "c++"
     host_objs = []
     for src in HOST_SOURCES:
@@ -67,7 +75,7 @@ def build(verbose=False):
     _run([cxx, "-shared", "-fPIC", "-o", hlib + ".tmp"] + host_objs +
          ["-L" + HERE, "-lqcjk", "-Wl,-rpath,$ORIGIN"])
     os.replace(hlib + ".tmp", hlib)
-    return lib, hlib
+    return lib, hlib, mlib
 
 
 if __name__ == "__main__":

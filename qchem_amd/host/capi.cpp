@@ -76,13 +76,14 @@ int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const dou
         p.fd_max = par->fd_max;
         p.sv_tol = par->sv_tol;
         p.verbose = par->verbose != 0;
+        p.guess = par->guess;
         Mat seed;
         if (D0) {
             seed = Mat(b.nbf);
             std::memcpy(seed.a.data(), D0, seed.a.size() * sizeof(double));
         }
         const int nelec = (int)std::lround(zsum) - charge;
-        SCFResult r = rhf(eng, atoms, nelec, p, D0 ? &seed : nullptr);
+        SCFResult r = rhf(eng, atoms, nelec, p, D0 ? &seed : nullptr, &b);
         res->energy = r.energy;
         res->e_nuc = r.e_nuc;
         res->e_one = r.e_one;
@@ -113,8 +114,8 @@ int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const dou
 }
 
 int qcscf_rhf_batch(const qchost_shells* mols, int32_t nmol, const int32_t* natom, const double* Z, const double* R,
-                    const int32_t* charge, const qcscf_params* par, int32_t nthreads, qcscf_result* res,
-                    double* e_hist, qcscf_batch_result* bres)
+                    const int32_t* charge, const qcscf_params* par, const double* D0, int32_t nthreads,
+                    qcscf_result* res, double* e_hist, double* D_out, qcscf_batch_result* bres)
 {
     try {
         if (!mols || nmol < 1 || !natom || !Z || !R || !par || !res) throw std::runtime_error("qcscf_rhf_batch: null argument");
@@ -148,11 +149,26 @@ int qcscf_rhf_batch(const qchost_shells* mols, int32_t nmol, const int32_t* nato
         p.nproj = par->nproj;
         p.fd_max = par->fd_max;
         p.sv_tol = par->sv_tol;
-        p.verbose = false;
+        p.verbose = par->verbose != 0;
+        p.guess = par->guess;
+        std::vector<Mat> seeds;
+        if (D0) {
+            size_t off = 0;
+            for (const Basis& b : bases) {
+                Mat s(b.nbf);
+                std::memcpy(s.a.data(), D0 + off, s.a.size() * sizeof(double));
+                off += s.a.size();
+                seeds.push_back(std::move(s));
+            }
+        }
         const BatchSCFResult r = rhf_batch(bp, atoms, nelec, p, o, par->jk_flags ? par->jk_flags : (QCJK_J | QCJK_K),
-                                           nthreads);
+                                           nthreads, D0 ? &seeds : nullptr);
         int nconv = 0;
+        size_t doff = 0;
         for (int m = 0; m < nmol; ++m) {
+            if (D_out && r.mols[m].D.n == bases[m].nbf)
+                std::memcpy(D_out + doff, r.mols[m].D.a.data(), r.mols[m].D.a.size() * sizeof(double));
+            doff += (size_t)bases[m].nbf * bases[m].nbf;
             const SCFResult& x = r.mols[m];
             qcscf_result& y = res[m];
             std::memset(&y, 0, sizeof(y));

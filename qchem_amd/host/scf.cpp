@@ -8,6 +8,7 @@
 #include <cstring>
 #include <deque>
 #include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 
@@ -150,7 +151,115 @@ Mat density_from(const Ortho& X, const Mat& Fp, int nocc, std::vector<double>& e
     return D;
 }
 
+// D = V C' diag(occ) C'^T V^T over the orthonormal eigenvectors C' of F'.
+Mat density_occ(const Ortho& X, const Mat& Cp, const std::vector<double>& occ)
+{
+    const int n = X.V.n, m = X.m;
+    Mat D(n);
+    std::vector<double> col(n);
+    for (size_t k = 0; k < occ.size(); ++k) {
+        if (occ[k] == 0.0) continue;
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < m; ++j) s += X.V(i, j) * Cp(j, (int)k);
+            col[i] = s;
+        }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) D(i, j) += occ[k] * col[i] * col[j];
+    }
+    return D;
+}
+
+// Aufbau with the open shell averaged: orbitals within 1e-4 Eh of each other share the electrons
+// left for their group equally (spherical, fractional-occupation atom).
+std::vector<double> averaged_occupations(const std::vector<double>& eps, double nelec)
+{
+    std::vector<double> occ(eps.size(), 0.0);
+    double left = nelec;
+    for (size_t i = 0; i < eps.size() && left > 1e-12;) {
+        size_t j = i;
+        while (j < eps.size() && std::fabs(eps[j] - eps[i]) < 1e-4) ++j;
+        const double put = std::min(left, 2.0 * (double)(j - i));
+        for (size_t k = i; k < j; ++k) occ[k] = put / (double)(j - i);
+        left -= put;
+        i = j;
+    }
+    return occ;
+}
+
+// Spherically averaged neutral-atom density of one atom in its own shells: fractional-occupation
+// RHF (averaged open shell), damped fixed point.
+Mat atomic_density(const Basis& ab, const Atom& atom, const qcjk_opts& o)
+{
+    GpuHF_IBS eng(ab.view(), o);
+    Mat S, h;
+    one_electron(eng, {atom}, S, h);
+    Ortho X;
+    X.set(S);
+    std::vector<double> eps;
+    Mat Cp;
+    sym_eigen(X.to_ortho(h), eps, Cp);
+    Mat D = density_occ(X, Cp, averaged_occupations(eps, atom.Z));
+    const WholeSystemFock_IBS<double>& whole = eng;
+    for (int it = 0; it < 200; ++it) {
+        const Mat J = whole.MakeAOFock(D, false);
+        Mat F = axpby(1.0, h, 1.0, J);
+        F = axpby(1.0, F, -0.5, whole.MakeAOFock(D, true));
+        sym_eigen(X.to_ortho(F), eps, Cp);
+        const Mat Dn = density_occ(X, Cp, averaged_occupations(eps, atom.Z));
+        const double change = frob(axpby(1.0, Dn, -1.0, D));
+        D = axpby(0.5, Dn, 0.5, D);
+        if (change < 1e-7) break;
+    }
+    return D;
+}
+
 }  // namespace
+
+Mat sad_density(const Basis& b, const std::vector<Atom>& atoms, const qcjk_opts& o)
+{
+    Mat D(b.nbf);
+    std::vector<int> bf0(b.shells.size() + 1, 0);
+    for (size_t s = 0; s < b.shells.size(); ++s) bf0[s + 1] = bf0[s] + (b.shells[s].L + 1) * (b.shells[s].L + 2) / 2;
+    // (atom signature, density), kept for the process: an element with the same shells always gets
+    // the same atomic density, as from a tabulated SAD database
+    static std::mutex cache_mu;
+    static std::vector<std::pair<std::vector<double>, Mat>> cache;
+    for (const Atom& a : atoms) {
+        Basis ab;
+        std::vector<int> idx;  // molecular basis-function index of each atomic one
+        for (size_t s = 0; s < b.shells.size(); ++s) {
+            const Shell& sh = b.shells[s];
+            const double dx = sh.R[0] - a.R[0], dy = sh.R[1] - a.R[1], dz = sh.R[2] - a.R[2];
+            if (dx * dx + dy * dy + dz * dz > 1e-16) continue;
+            ab.shells.push_back(sh);
+            for (int k = bf0[s]; k < bf0[s + 1]; ++k) idx.push_back(k);
+        }
+        if (ab.shells.empty()) continue;
+        // same element with the same shells (translation-invariant): reuse the atomic density
+        std::vector<double> sig{a.Z};
+        for (const Shell& sh : ab.shells) {
+            sig.push_back(sh.L + 100.0 * sh.contracted);
+            sig.insert(sig.end(), sh.exps.begin(), sh.exps.end());
+            sig.insert(sig.end(), sh.coefs.begin(), sh.coefs.end());
+        }
+        Mat Da;
+        {
+            std::lock_guard<std::mutex> lk(cache_mu);
+            for (const auto& c : cache)
+                if (c.first == sig) Da = c.second;
+        }
+        if (Da.n == 0) {
+            ab.finalize();
+            Da = atomic_density(ab, a, o);
+            std::lock_guard<std::mutex> lk(cache_mu);
+            cache.emplace_back(sig, Da);
+        }
+        for (size_t i = 0; i < idx.size(); ++i)
+            for (size_t j = 0; j < idx.size(); ++j) D(idx[i], idx[j]) = Da((int)i, (int)j);
+    }
+    return D;
+}
 
 double nuclear_repulsion(const std::vector<Atom>& atoms)
 {
@@ -246,12 +355,20 @@ struct RHFRun {
     }
 };
 
-SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0)
+SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0,
+              const Basis* basis_for_sad)
 {
     const auto t_start = clk::now();
     const int n = eng.nbf();
     Mat S, h;
     one_electron(eng, atoms, S, h);
+    Mat sad;
+    if (!D0 && par.guess == 1) {
+        if (!basis_for_sad) throw std::runtime_error("rhf: the SAD seed needs the basis");
+        qcjk_opts o{};
+        sad = sad_density(*basis_for_sad, atoms, o);
+        D0 = &sad;
+    }
     RHFRun run;
     run.setup(S, h, atoms, nelec, par, D0);
     run.res.t_setup = secs(t_start, clk::now());
@@ -282,7 +399,7 @@ SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const S
 
 BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vector<std::vector<Atom>>& atoms,
                          const std::vector<int>& nelec, const SCFParams& par, const qcjk_opts& opts, unsigned jk_flags,
-                         int nthreads)
+                         int nthreads, const std::vector<Mat>* seeds)
 {
     const auto t_start = clk::now();
     const int nmol = (int)bases.size();
@@ -338,7 +455,16 @@ BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vect
         if (rc != QCJK_OK) throw EngineError("qcjk_one_electron_mol", rc);
         h1[m] = axpby(1.0, T, 1.0, V);
     }
-    parallel([&](int m) { runs[m].setup(S[m], h1[m], atoms[m], nelec[m], par, nullptr); });
+    std::vector<Mat> sad(nmol);
+    if (!seeds && par.guess == 1) {
+        qcjk_opts ao{};
+        ao.device = opts.device;
+        for (int m = 0; m < nmol; ++m) sad[m] = sad_density(*bases[m], atoms[m], ao);
+    }
+    parallel([&](int m) {
+        const Mat* d0 = seeds ? &(*seeds)[m] : (par.guess == 1 ? &sad[m] : nullptr);
+        runs[m].setup(S[m], h1[m], atoms[m], nelec[m], par, d0);
+    });
     out.t_setup = secs(t_start, clk::now());
 
     std::vector<double> Dp(nelem), Jp(nelem), Kp(nelem);

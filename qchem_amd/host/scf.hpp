@@ -35,6 +35,7 @@ struct SCFParams {
     double fd_max = -1.0;    // < 0: Z^2 0.1/32
     double sv_tol = 5e-9;    // DIISParams SVTol
     bool verbose = false;
+    int guess = 0;           // 0 core Hamiltonian, 1 SAD (used when no explicit seed is given)
 };
 
 struct SCFResult {
@@ -54,8 +55,15 @@ double nuclear_repulsion(const std::vector<Atom>& atoms);
 // One-electron h = T + V and S through the engine.
 void one_electron(GpuHF_IBS& eng, const std::vector<Atom>& atoms, Mat& S, Mat& h);
 
+// Superposition of atomic densities (the SAD seed): each atom's spherically averaged, fractional-
+// occupation neutral-atom density in the atom's own shells, placed block-diagonally.  This is the
+// matrix form of the seed; the reference's SAD (ChargeDensity/AtomicDensity.C) reads tabulated
+// radial densities instead.  Atoms are matched to shells by centre.
+Mat sad_density(const Basis& b, const std::vector<Atom>& atoms, const qcjk_opts& o);
+
 // RHF to convergence.  D0 (optional) is an explicit seed density; otherwise the core guess.
-SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0 = nullptr);
+SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0 = nullptr,
+              const Basis* basis_for_sad = nullptr);
 
 // Many independent molecules, each running the rhf() recipe, with ONE batched J/K build per
 // iteration for all of them (qcjk_create_batch; the build replays a captured hipGraph).  A
@@ -70,6 +78,6 @@ struct BatchSCFResult {
 };
 BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vector<std::vector<Atom>>& atoms,
                          const std::vector<int>& nelec, const SCFParams& par, const qcjk_opts& opts, unsigned jk_flags,
-                         int nthreads);
+                         int nthreads, const std::vector<Mat>* seeds = nullptr);
 
 }  // namespace qchem_amd

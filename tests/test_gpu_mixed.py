@@ -99,9 +99,9 @@ def test_heavy_v1_stays_fp64(require_gpu):
 def test_mixed_rhf_energy(require_gpu):
     m = S.acetone_like()
     B = qa.Basis(m["shells"])
-    r64 = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7)
+    r64 = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7, guess=qa.GUESS_SAD)
     rmx = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7, jk_flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED,
-                 mixed_tol=1e-7)
+                 mixed_tol=1e-7, guess=qa.GUESS_SAD)
     print("RHF fp64 %.10f (%d it) mixed %.10f (%d it) dE %.2e" % (
         r64["energy"], r64["iterations"], rmx["energy"], rmx["iterations"], rmx["energy"] - r64["energy"]))
     assert r64["converged"] and rmx["converged"]
