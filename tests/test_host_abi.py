@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"\b(qc(?:jk|host|scf)_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(qc(?:jk|host|scf|ml)_[a-z_]+)\s*\(", text)))
 
 
 def test_engine_exports_every_declared_symbol():
@@ -37,6 +37,25 @@ def test_host_exports_every_declared_symbol():
     assert names and set(names) <= set(qa.HOST_SYMBOLS)
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_network_exports_every_declared_symbol():
+    from qchem_amd import mlp
+    lib = ctypes.CDLL(mlp.lib_path())
+    names = _declared("qcml.h")
+    assert names and set(names) <= set(mlp.QCML_SYMBOLS)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_network_needs_a_device():
+    from conftest import gpu_available
+    from qchem_amd import mlp
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(mlp.NetError) as e:
+        mlp.Net(4, 8, 2, max_batch=4)
+    assert e.value.code == 5
 
 
 def test_strerror_codes():
