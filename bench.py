@@ -54,12 +54,12 @@ def cpu_baseline(mol, D, budget_s=15.0, threads=None):
 def time_to(e_hist, t_hist, e_ref, tol=1e-6):
     for e, t in zip(e_hist, t_hist):
         if abs(e - e_ref) < tol:
-            return t
+            return float(t)
     return None
 
 
 def scf_gpu(qa, mol, **kw):
-    """FP64 GPU RHF (C++ driver through the HF faces) converged tightly; time-to-1e-6 against itself."""
+    """GPU RHF (C++ driver through the HF faces) converged tightly; time-to-1e-6 against itself."""
     B = qa.Basis(mol["shells"])
     res = qa.rhf(B, mol["Z"], mol["R"], e_tol=1e-10, fd_tol=1e-7, **kw)
     return {"energy": res["energy"], "iterations": res["iterations"], "converged": res["converged"],
@@ -83,13 +83,86 @@ def scf_cpu_reference(mol, e_conv):
             "delta_e_vs_gpu": res["energy"] - e_conv}
 
 
+def scf_time_to_solution(qa, S, geometries=(100000, 100001, 100002, 100003), mixed_tol=1e-7):
+    """Held-out C3H6O-like geometries: time to |E - E_conv| < 1e-6 Eh for the plain FP64 GPU path
+    (SAD seed) and for mixed precision + the learned seed; E_conv from the FP64 run."""
+    from qchem_amd import guess as G
+    if not os.path.exists(G.DEFAULT_MODEL):
+        return {"skipped": "no trained model at %s" % os.path.relpath(G.DEFAULT_MODEL, ROOT)}
+    model = G.LearnedGuess.load()
+    rows = G.evaluate(model, list(geometries), log=lambda *_: None, mixed_tol=mixed_tol)
+    summ = G.summarize(rows)
+    summ["geometries"] = list(geometries)
+    summ["first_energy_error_learned_eh"] = [r["learned_fp64"]["first_energy_error"] for r in rows]
+    return summ
+
+
+def mixed_precision_chain(qa, S, mixed_tol=1e-7):
+    """C16H32-like chain (784 functions): FP64 vs mixed J/K for one density, errors and build times."""
+    import numpy as np
+    import torch
+    m = S.alkane_chain()
+    B = qa.Basis(m["shells"])
+    eng = qa.JKEngine(B, mixed_tol=mixed_tol)
+    from qchem_amd import tune
+    tune.apply(eng, workload="random-cluster-16")
+    n = B.nbf
+    rng = np.random.default_rng(11)
+    C = rng.normal(size=(n, sum(m["Z"]) // 2)) / np.sqrt(n)
+    D = torch.from_numpy(2.0 * C @ C.T).cuda()
+    out = {"workload": m["name"], "n_basis": n, "mixed_tol": mixed_tol}
+    res = {}
+    for name, fl in (("fp64", qa.QCJK_J | qa.QCJK_K), ("mixed", qa.QCJK_J | qa.QCJK_K | qa.QCJK_MIXED)):
+        J, K = torch.empty_like(D), torch.empty_like(D)
+        st = eng.build_device(D, J, K, fl, stats=True)
+        res[name] = (J, K, st)
+        out[name + "_build_s"] = st.device_ms / 1e3
+        out[name + "_quartets"] = st.quartets_computed
+    J0, K0, _ = res["fp64"]
+    J1, K1, st1 = res["mixed"]
+    out["fp32_fraction"] = st1.quartets_fp32 / max(1, st1.quartets_computed)
+    out["J_max_rel_err"] = float((J1 - J0).abs().max() / J0.abs().max())
+    out["K_max_rel_err"] = float((K1 - K0).abs().max() / K0.abs().max())
+    out["delta_E_two_electron_eh"] = float((0.5 * (D * (J1 - J0)).sum() - 0.25 * (D * (K1 - K0)).sum()).item())
+    return out
+
+
+def batched_builds(qa, S, nmol=512, reps=3):
+    """512 molecules of 2-12 atoms in one handle: one launch per class per build, graph-replayed."""
+    import numpy as np
+    import torch
+    mols = S.batch(nmol)
+    bases = [qa.Basis(m["shells"]) for m in mols]
+    t0 = time.perf_counter()
+    eng = qa.JKEngine(bases)
+    t_create = time.perf_counter() - t0
+    rng = np.random.default_rng(5)
+    Ds = []
+    for b, m in zip(bases, mols):
+        C = rng.normal(size=(b.nbf, max(1, sum(m["Z"]) // 2))) / np.sqrt(b.nbf)
+        Ds.append(2.0 * C @ C.T)
+    D = torch.from_numpy(eng.pack(Ds)).cuda()
+    J, K = torch.empty_like(D), torch.empty_like(D)
+    st = eng.build_device(D, J, K, stats=True)
+    eng.graph_build(D, J, K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.graph_build(D, J, K)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"molecules": nmol, "n_basis_range": [min(eng.nbfs), max(eng.nbfs)], "n_basis_total": sum(eng.nbfs),
+            "create_s": t_create, "quartets_per_build": st.quartets_computed, "build_s": dt,
+            "quartets_per_s": st.quartets_computed / dt, "per_molecule_build_ms": 1e3 * dt / nmol}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-scf", action="store_true", help="skip the SCF time-to-solution section")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF, mixed-precision and batched sections")
     ap.add_argument("--no-tune", action="store_true", help="use the untuned default kernel variant per class")
     ap.add_argument("--retune", action="store_true", help="re-run the per-class kernel-variant autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -113,6 +186,7 @@ def main():
     import numpy as np
     import torch
 
+    torch.zeros(1, device="cuda:0")  # torch's HIP runtime up first: it is the device-memory plumbing below
     import qchem_amd as qa
     from qchem_amd import systems as S
 
@@ -173,12 +247,16 @@ def main():
             "build_tflops": total_flops / (ms / 1e3) / 1e12}
 
     cpu = None if args.no_cpu else cpu_baseline(mol, Dh)
-    scf = None
+    scf = mixed = batched = None
     if not args.no_scf:
         n2 = S.n2_even_tempered()
-        scf = {"n2_even_tempered_fp64_gpu": scf_gpu(qa, n2), "random_cluster_fp64_gpu": scf_gpu(qa, mol)}
+        scf = {"n2_even_tempered_fp64_gpu_core_guess": scf_gpu(qa, n2),
+               "c3h6o_time_to_1e-6": scf_time_to_solution(qa, S)}
         if cpu is not None:
-            cpu["scf_n2_reference_algorithm"] = scf_cpu_reference(n2, scf["n2_even_tempered_fp64_gpu"]["energy"])
+            cpu["scf_n2_reference_algorithm"] = scf_cpu_reference(
+                n2, scf["n2_even_tempered_fp64_gpu_core_guess"]["energy"])
+        mixed = mixed_precision_chain(qa, S)
+        batched = batched_builds(qa, S)
     out = {"metric": METRIC, "value": value, "unit": "quartets/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded random cluster, random PSD density)",
@@ -186,7 +264,7 @@ def main():
                                   % (mol["name"], n, quartets),
                       "n_basis": n, "quartets_per_build": quartets, "fock_build_s": ms / 1e3,
                       "parallelism": "single GPU"},
-           "roofline": roof, "cpu_baseline": cpu, "scf": scf,
+           "roofline": roof, "cpu_baseline": cpu, "scf": scf, "mixed_precision": mixed, "batched": batched,
            "tuning": tuning,
            "classes": [{k: r[k] for k in ("la", "lb", "lc", "ld", "wavefront_per_quartet", "quartets")} |
                        {"ms": r["device_ms"] / max(1, r["launches"]),

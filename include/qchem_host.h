@@ -80,6 +80,11 @@ int qcscf_rhf_batch(const qchost_shells* mols, int32_t nmol, const int32_t* nato
                     const int32_t* charge, const qcscf_params* par, const double* D0, int32_t nthreads,
                     qcscf_result* res, double* e_hist, double* D_out, qcscf_batch_result* bres);
 
+/* The SAD seed density (nbf x nbf, row-major) of a molecule: each atom's spherically averaged
+ * fractional-occupation neutral-atom density in its own shells, placed block-diagonally.  Atomic
+ * densities are cached per element and shell set for the life of the process. */
+int qcscf_sad(const qchost_shells* sh, int32_t natom, const double* Z, const double* R, int32_t device, double* D_out);
+
 /* Last error message of this thread (exception text), or "". */
 const char* qchost_last_error(void);
 

@@ -92,7 +92,7 @@ QCJK_SYMBOLS = ["qcjk_create", "qcjk_create_batch", "qcjk_nbf", "qcjk_layout", "
                 "qcjk_graph_build", "qcjk_buffers", "qcjk_one_electron", "qcjk_one_electron_mol", "qcjk_schwarz",
                 "qcjk_profile", "qcjk_class_report", "qcjk_set_variant", "qcjk_autotune", "qcjk_destroy",
                 "qcjk_strerror"]
-HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qcscf_rhf_batch", "qchost_last_error"]
+HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qcscf_rhf_batch", "qcscf_sad", "qchost_last_error"]
 
 
 class _BatchResult(ctypes.Structure):
@@ -172,6 +172,8 @@ def host_lib():
                                       ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
                                       ctypes.POINTER(_ScfResult), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_BatchResult)]
+        H.qcscf_sad.argtypes = [ctypes.POINTER(_Shells), ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double), ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         H.qchost_last_error.restype = ctypes.c_char_p
         _HOST = H
     return _HOST
@@ -391,6 +393,17 @@ def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, f
             "energy_history": eh[:it].copy(), "time_history": th[:it].copy(), "t_total": res.t_total,
             "t_jk": res.t_jk, "t_diag": res.t_diag, "t_setup": res.t_setup, "quartets": res.quartets,
             "fp32_fraction": res.quartets_fp32 / max(1, res.quartets)}
+
+
+def sad_density(basis, Z, R, device=0):
+    """SAD seed density of a molecule (qcscf_sad)."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+    D = np.zeros((basis.nbf, basis.nbf))
+    rc = host_lib().qcscf_sad(ctypes.byref(basis._shells()), len(Z), _dp(Z), _dp(R), device, _dp(D))
+    if rc != 0:
+        raise EngineError("qcscf_sad", rc, host_lib().qchost_last_error().decode())
+    return D
 
 
 def rhf_batch(bases, mols, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_max=-1.0, sv_tol=5e-9,

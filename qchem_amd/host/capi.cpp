@@ -210,6 +210,30 @@ int qcscf_rhf_batch(const qchost_shells* mols, int32_t nmol, const int32_t* nato
     }
 }
 
+int qcscf_sad(const qchost_shells* sh, int32_t natom, const double* Z, const double* R, int32_t device, double* D_out)
+{
+    try {
+        if (!sh || natom < 1 || !Z || !R || !D_out) throw std::runtime_error("qcscf_sad: null argument");
+        Basis b = make_basis(sh);
+        std::vector<Atom> atoms(natom);
+        for (int i = 0; i < natom; ++i) {
+            atoms[i].Z = Z[i];
+            for (int k = 0; k < 3; ++k) atoms[i].R[k] = R[3 * i + k];
+        }
+        qcjk_opts o{};
+        o.device = device;
+        const Mat D = sad_density(b, atoms, o);
+        std::memcpy(D_out, D.a.data(), D.a.size() * sizeof(double));
+        return 0;
+    } catch (const EngineError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 100;
+    }
+}
+
 const char* qchost_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"

@@ -161,3 +161,20 @@ def test_autotune_keeps_results(require_gpu):
     assert rep and all(r["device_ms"] > 0 for r in rep)
     J1, K1, _ = eng.build(D)
     assert rel_err(J1, J0) < 1e-12 and rel_err(K1, K0) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["water", "N2"])
+def test_dzvp_anchor_rhf(require_gpu, name):
+    """The C++ RHF over the engine on the reference's HF/DZVP anchor systems: the reference's own
+    energy (at its test tolerance) and the oracle's SCF (1e-8 Eh)."""
+    from qchem_amd import basisfile as bf
+    from test_oracle import ANCHORS, DZVP
+    Z, R, e_ref, rel = ANCHORS[name]
+    shells = bf.molecule_shells(DZVP, Z, R)
+    res = qa.rhf(qa.Basis(shells), Z, R, e_tol=1e-10, fd_tol=1e-7)
+    ob = oracle.Basis(shells)
+    ref = oscf.rhf(ob, Z, R, lambda D: ob.jk_direct(D)[:2], e_tol=1e-10, fd_tol=1e-7)
+    print("%s DZVP: GPU %.9f oracle %.9f reference anchor %.6f" % (name, res["energy"], ref["energy"], e_ref))
+    assert res["converged"]
+    assert abs(res["energy"] - ref["energy"]) < 1e-8
+    assert abs(res["energy"] - e_ref) < rel * abs(e_ref)

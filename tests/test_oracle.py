@@ -182,3 +182,26 @@ def test_oracle_scf_n2():
     r = oscf.rhf(ob, m["Z"], m["R"], lambda D: ob.jk_direct(D)[:2])
     assert r["converged"]
     assert abs(r["energy"] - (-106.9323789865)) < 1e-8
+
+
+DZVP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dzvp_HNO.bsd")
+# The reference's HF/DZVP anchors (IntegrationTests/M_HF_U.C:21-51): geometry in bohr, energy in Eh,
+# and the relative tolerance that test uses.
+ANCHORS = {
+    "water": ([8, 1, 1], [[0, 0, 0], [0, 1.431, 1.107], [0, -1.431, 1.107]], -76.022903, 1e-5),
+    "N2": ([7, 7], [[-1.03, 0, 0], [1.04, 0, 0]], -108.94524, 1e-4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(ANCHORS))
+def test_oracle_scf_dzvp_anchor(name):
+    """Basis reader + integrals + normalisation + SCF against the reference's own converged energies.
+    The anchors are printed to 6 and 5 decimals; the oracle lands within their rounding."""
+    import oracle.scf as oscf
+    from qchem_amd import basisfile as bf
+    Z, R, e_ref, rel = ANCHORS[name]
+    ob = oracle.Basis(bf.molecule_shells(DZVP, Z, R))
+    r = oscf.rhf(ob, Z, R, lambda D: ob.jk_direct(D)[:2], e_tol=1e-10, fd_tol=1e-7)
+    assert r["converged"]
+    assert abs(r["energy"] - e_ref) < rel * abs(e_ref)
+    assert abs(r["energy"] - e_ref) < 5e-6  # the anchor's last printed digit
