@@ -51,6 +51,20 @@ def cpu_baseline(mol, D, budget_s=15.0, threads=None):
                       "(%d contracted quartets, %.1f s)" % (limit2, npairs, cnt, dt)}
 
 
+def profiled_traffic(kernel_prefix):
+    """HBM bytes per dispatch of the kernel (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
+    committed rocprofv3 summary of this workload, or None when it has no entry for the kernel."""
+    path = os.path.join(ROOT, "profiles", "jk_build_random_cluster_fp64.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_JSON:"):
+            table = json.loads(line.split(":", 1)[1])
+            hits = [v for k, v in table.items() if k.startswith(kernel_prefix)]
+            return hits[0] if hits else None
+    return None
+
+
 def time_to(e_hist, t_hist, e_ref, tol=1e-6):
     for e, t in zip(e_hist, t_hist):
         if abs(e - e_ref) < tol:
@@ -239,10 +253,11 @@ def main():
     per_launch_s = dom["device_ms"] / launches / 1e3
     achieved = per_launch_flops / per_launch_s / 1e12
     total_flops = sum(r["flops"] for r in rep) / max(1, args.steps)
+    kname = "k_jk_%s<%d, %d, %d, %d," % (VARIANT_KERNEL[dom["wavefront_per_quartet"]], dom["la"], dom["lb"],
+                                          dom["lc"], dom["ld"])
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-            "kernel": "k_jk_%s<%d,%d,%d,%d>" % (VARIANT_KERNEL[dom["wavefront_per_quartet"]],
-                                                dom["la"], dom["lb"], dom["lc"], dom["ld"]),
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(kname),
+            "kernel": kname.rstrip(",") + ">",
             "kernel_ms_per_launch": per_launch_s * 1e3, "kernel_flops_per_launch": per_launch_flops,
             "build_tflops": total_flops / (ms / 1e3) / 1e12}
 

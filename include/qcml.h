@@ -6,8 +6,8 @@
  * bias/activation, MSE, Adam).  FP32 master weights and Adam moments stay on the device; the BF16
  * copies the GEMMs read are refreshed after every update.  Row-major everywhere: X is B x nin, W1 is
  * nin x width, W2 width x width, W3 width x nout, outputs B x nout.  One net = one host thread at a
- * time; every call is asynchronous on `stream` (null = the net's own stream) unless it returns
- * host data.
+ * time; every call is asynchronous on `stream` exactly as given (null = the default stream, so a
+ * caller on the default stream is ordered with the net) unless it returns host data.
  */
 #ifndef QCML_H
 #define QCML_H

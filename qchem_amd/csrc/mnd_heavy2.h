@@ -39,13 +39,6 @@ struct H2Smem {
     RT X[C::IB][S::NCD][S::NHAB];
 };
 
-// FP64 and FP32 quartets of one kernel share the LDS.
-template <int LA, int LB, int LC, int LD>
-union H2Union {
-    H2Smem<LA, LB, LC, LD, double> d;
-    H2Smem<LA, LB, LC, LD, float> f;
-};
-
 // Build the per-entry R recursion table (once per kernel): dir | i1<<2 | i2<<10 | c2<<18.
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void heavy2_init(int* rtab, int lane)

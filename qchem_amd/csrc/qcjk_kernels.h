@@ -20,7 +20,10 @@
 //   2 k_jk_heavy2  wavefront per quartet, register-resident transforms, batched R (mnd_heavy2.h).
 //   3 k_jk_lightc  lane per quartet, chunked over the ket's first component.
 // With QCJK_MIXED, variants 0, 2 and 3 evaluate small quartets' primitive integrals in FP32
-// (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.
+// (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.  The FP32 work
+// runs in its own instantiation of the kernel (FP32 = true), launched after the FP64 one over the
+// same work list: each takes only the units of its precision, so each keeps its own register
+// allocation and neither carries the other's code.
 #pragma once
 
 #include <algorithm>
@@ -42,6 +45,33 @@ __device__ __forceinline__ int find_unit_owner(const long long* __restrict__ sta
     return lo;
 }
 
+// Wavefront kernels (one quartet per unit): each block takes a contiguous run of units, so the
+// owning bra pair is found by one binary search and then followed forward (a unit's bra is the
+// previous unit's or a later one), instead of one dependent search per quartet.  Within a class
+// every quartet has the same shape, so equal runs balance.
+struct UnitRange {
+    long long u, end, next;
+    int P;
+    __device__ __forceinline__ explicit UnitRange(const ClassPairArgs& C)
+    {
+        const long long per = (C.nunits + gridDim.x - 1) / gridDim.x;
+        u = (long long)blockIdx.x * per;
+        end = u + per < C.nunits ? u + per : C.nunits;
+        P = 0;
+        next = 0;
+        if (u < end) {
+            P = find_unit_owner(C.unit_start, C.nbra, u);
+            next = C.unit_start[P + 1];
+        }
+    }
+    // owner of unit v (v increases by one per call, v < nunits = unit_start[nbra])
+    __device__ __forceinline__ int owner(const ClassPairArgs& C, long long v)
+    {
+        while (v >= next) next = C.unit_start[++P + 1];
+        return P;
+    }
+};
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -49,10 +79,10 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Mixed precision, lane-per-quartet kernels: the wavefront runs its primitive integrals in FP32
-// when every active lane's quartet has Q_ab Q_cd max|D'| below mixed_tol (one ballot, so the
-// choice is uniform and the two paths never diverge within a wavefront).  Ket pairs are sorted by
-// descending bound, so the choice is almost always the per-lane one.
+// Mixed precision, lane-per-quartet kernels: a wavefront's unit belongs to the FP32 kernel when
+// every active lane's quartet has Q_ab Q_cd max|D'| below mixed_tol (one ballot, so the choice is
+// uniform over the wavefront).  Ket pairs are sorted by descending bound, so the choice is almost
+// always the per-lane one.
 __device__ __forceinline__ bool wave_fp32(const BuildArgs& A, bool active, double4 pg, double4 qg)
 {
     if (!(A.flags & kFlagMixed)) return false;
@@ -75,13 +105,13 @@ __device__ __forceinline__ void count_lane_quartets(const BuildArgs& A, bool act
 // ---------------------------------------------------------------------------------------------
 // Lane-per-quartet kernel
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, bool BRA_MEM>
+template <int LA, int LB, int LC, int LD, bool BRA_MEM, bool FP32>
 __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
-    __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
-    __shared__ HermiteAxis<LA, LB, float> sEab32[BRA_MEM ? 4 : 1][3];
+    using RT = std::conditional_t<FP32, float, double>;
+    __shared__ HermiteAxis<LA, LB, RT> sEab[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
     for (long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); unit < C.nunits;
          unit += (long long)gridDim.x * 4) {  // uniform per wavefront
@@ -100,16 +130,16 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
         const int4 qi = A.pinfo[C.ket_off + Q];
         const double4 qg = A.pgeo[C.ket_off + Q];
         if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
-        const bool fp32 = wave_fp32(A, active, pg, qg);
+        if (wave_fp32(A, active, pg, qg) != FP32) continue;  // the other precision's unit
         const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
 
         double I[NAB * NCD];
         static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = 0.0; });
         if (active) {
-            if (fp32) {
+            if constexpr (FP32) {
                 float If[NAB * NCD];
                 static_for<0, NAB * NCD>([&](auto i) { If[decltype(i)::value] = 0.0f; });
-                light_compute<LA, LB, LC, LD, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
+                light_compute<LA, LB, LC, LD, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab[slot]);
                 static_for<0, NAB * NCD>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
             } else {
                 light_compute<LA, LB, LC, LD, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
@@ -217,7 +247,7 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
                 });
             }
         }
-        if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
+        if (A.counters) count_lane_quartets(A, active, FP32, lane, pi, qi);
     }
 }
 
@@ -226,13 +256,13 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
 // per chunk (R recomputed), so only NAB*ND accumulators are live.  J_ab, K_ad, K_bd accumulate
 // across chunks; J_cd, K_ac, K_bc of a chunk are final when it ends.
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, bool BRA_MEM>
+template <int LA, int LB, int LC, int LD, bool BRA_MEM, bool FP32>
 __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB;
-    __shared__ HermiteAxis<LA, LB, double> sEab[BRA_MEM ? 4 : 1][3];
-    __shared__ HermiteAxis<LA, LB, float> sEab32[BRA_MEM ? 4 : 1][3];
+    using RT = std::conditional_t<FP32, float, double>;
+    __shared__ HermiteAxis<LA, LB, RT> sEab[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
     for (long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); unit < C.nunits;
          unit += (long long)gridDim.x * 4) {  // uniform per wavefront
@@ -251,7 +281,7 @@ __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
         const int4 qi = A.pinfo[C.ket_off + Q];
         const double4 qg = A.pgeo[C.ket_off + Q];
         if (A.flags & kFlagScreen) active = active && (pg.w * qg.w * (*A.dmax) >= A.screen_tol);
-        const bool fp32 = wave_fp32(A, active, pg, qg);
+        if (wave_fp32(A, active, pg, qg) != FP32) continue;  // the other precision's unit
         const int slot = BRA_MEM ? (threadIdx.x >> 6) : 0;
         double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
         if (!active) s = 0.0;
@@ -266,10 +296,10 @@ __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
             double I[NAB * ND];
             static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = 0.0; });
             if (active) {
-                if (fp32) {
+                if constexpr (FP32) {
                     float If[NAB * ND];
                     static_for<0, NAB * ND>([&](auto i) { If[decltype(i)::value] = 0.0f; });
-                    light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab32[slot]);
+                    light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, float>(A, pi, pg, qi, qg, If, sEab[slot]);
                     static_for<0, NAB * ND>([&](auto i) { I[decltype(i)::value] = (double)If[decltype(i)::value]; });
                 } else {
                     light_compute_chunk<LA, LB, LC, LD, c, BRA_MEM, double>(A, pi, pg, qi, qg, I, sEab[slot]);
@@ -356,7 +386,7 @@ __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
                 unsafeAtomicAdd(&Kt[(ob + b) * n + od + d], s * kbd[decltype(x)::value]);
             });
         }
-        if (A.counters) count_lane_quartets(A, active, fp32, lane, pi, qi);
+        if (A.counters) count_lane_quartets(A, active, FP32, lane, pi, qi);
     }
 }
 
@@ -374,8 +404,9 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
     HeavyLane<LA, LB, LC, LD> ln;
     ln.init(lane);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
-    for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
-        const int P = find_unit_owner(C.unit_start, C.nbra, u);
+    UnitRange ur(C);
+    for (long long u = ur.u; u < ur.end; ++u) {
+        const int P = ur.owner(C, u);
         const int Q = C.kseg[P].x + (int)(u - C.unit_start[P]);
         const int2 mm = A.pmat[C.bra_off + P];
         const int4 pi = A.pinfo[C.bra_off + P];
@@ -403,21 +434,23 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool FP32>
 __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 {
     using Dg = HeavyDigest<LA, LB, LC, LD>;
     using S = HeavyShape<LA, LB, LC, LD>;
+    using RT = std::conditional_t<FP32, float, double>;
     __shared__ int rtab[S::NR];
     __shared__ double sI[S::NI];
-    __shared__ H2Union<LA, LB, LC, LD> sm;
+    __shared__ H2Smem<LA, LB, LC, LD, RT> sm;
     const int lane = threadIdx.x;
     heavy2_init<LA, LB, LC, LD>(rtab, lane);
     int t0 = 0, u0 = 0, v0 = 0;
     if (lane < S::NHAB) hdec(lane, t0, u0, v0);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
-    for (long long u = blockIdx.x; u < C.nunits; u += gridDim.x) {
-        const int P = find_unit_owner(C.unit_start, C.nbra, u);
+    UnitRange ur(C);
+    for (long long u = ur.u; u < ur.end; ++u) {
+        const int P = ur.owner(C, u);
         const int Q = C.kseg[P].x + (int)(u - C.unit_start[P]);
         const int2 mm = A.pmat[C.bra_off + P];
         const int4 pi = A.pinfo[C.bra_off + P];
@@ -425,9 +458,9 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
         const int4 qi = A.pinfo[C.ket_off + Q];
         const double4 qg = A.pgeo[C.ket_off + Q];
         if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
-        const bool fp32 = (A.flags & kFlagMixed) && pg.w * qg.w * (*A.dmax) < A.mixed_tol;  // one quartet per wave
-        if (fp32) heavy2_compute<LA, LB, LC, LD, float>(A, sm.f, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
-        else heavy2_compute<LA, LB, LC, LD, double>(A, sm.d, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        const bool small = (A.flags & kFlagMixed) && pg.w * qg.w * (*A.dmax) < A.mixed_tol;  // one quartet per wave
+        if (small != FP32) continue;  // the other precision's quartet
+        heavy2_compute<LA, LB, LC, LD, RT>(A, sm, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
         const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
         for (int t = lane; t < Dg::NTASK; t += 64) {
             int row, col;
@@ -439,7 +472,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
         }
         if (A.counters && lane == 0) {
             atomicAdd(&A.counters[0], 1ull);
-            if (fp32) atomicAdd(&A.counters[1], 1ull);
+            if (FP32) atomicAdd(&A.counters[1], 1ull);
             atomicAdd(&A.counters[2], (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
         }
         __syncthreads();  // digest reads of sI finish before the next quartet clears it
@@ -458,7 +491,11 @@ hipError_t launch_light(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
 {
     if (c.nunits <= 0) return hipSuccess;
     const long long blocks = std::min<long long>((c.nunits + 3) / 4, kLaneGridCap);
-    hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
+    hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD, bra_in_lds(LA, LB), false>), dim3((unsigned)blocks), dim3(256), 0, s,
+                       a, c);
+    if (a.flags & kFlagMixed)
+        hipLaunchKernelGGL((k_jk_light<LA, LB, LC, LD, bra_in_lds(LA, LB), true>), dim3((unsigned)blocks), dim3(256), 0,
+                           s, a, c);
     return hipGetLastError();
 }
 
@@ -467,7 +504,11 @@ hipError_t launch_lightc(const BuildArgs& a, const ClassPairArgs& c, hipStream_t
 {
     if (c.nunits <= 0) return hipSuccess;
     const long long blocks = std::min<long long>((c.nunits + 3) / 4, kLaneGridCap);
-    hipLaunchKernelGGL((k_jk_lightc<LA, LB, LC, LD, bra_in_lds(LA, LB)>), dim3((unsigned)blocks), dim3(256), 0, s, a, c);
+    hipLaunchKernelGGL((k_jk_lightc<LA, LB, LC, LD, bra_in_lds(LA, LB), false>), dim3((unsigned)blocks), dim3(256), 0,
+                       s, a, c);
+    if (a.flags & kFlagMixed)
+        hipLaunchKernelGGL((k_jk_lightc<LA, LB, LC, LD, bra_in_lds(LA, LB), true>), dim3((unsigned)blocks), dim3(256),
+                           0, s, a, c);
     return hipGetLastError();
 }
 
@@ -487,7 +528,9 @@ hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t
     if (c.nunits <= 0) return hipSuccess;
     const long long cap = 256LL * 64;
     const long long blocks = c.nunits < cap ? c.nunits : cap;
-    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    if (a.flags & kFlagMixed)
+        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
     return hipGetLastError();
 }
 

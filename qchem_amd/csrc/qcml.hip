@@ -176,7 +176,6 @@ hipError_t gemm(int M, int N, int K, const void* A, bool abf, long long sam, lon
 
 struct qcml_net {
     int device = 0;
-    hipStream_t stream = nullptr;
     int nin = 0, width = 0, nout = 0, maxb = 0;
     long long nparam = 0;
     long long off[6] = {};   // W1 b1 W2 b2 W3 b3 within the flat arrays
@@ -199,11 +198,10 @@ void free_net(qcml_net* n)
 {
     if (!n) return;
     hipSetDevice(n->device);
-    if (n->stream) hipStreamSynchronize(n->stream);
+    hipDeviceSynchronize();
     void* p[] = {n->w, n->g, n->m, n->v, n->wb, n->Z1, n->H1, n->Z2, n->H2, n->Y, n->dY, n->dH2, n->dH1, n->partial};
     for (void* x : p)
         if (x) hipFree(x);
-    if (n->stream) hipStreamDestroy(n->stream);
     delete n;
 }
 
@@ -299,8 +297,7 @@ int qcml_create(int32_t nin, int32_t width, int32_t nout, int32_t max_batch, uin
     }
     n->nparam = o;
     auto fail = [&]() { free_net(n); return 3; };
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail();
+    if (hipSetDevice(device) != hipSuccess) return fail();
     const long long bh = (long long)max_batch * width, bo = (long long)max_batch * nout;
     if (dalloc(&n->w, o) || dalloc(&n->g, o) || dalloc(&n->m, o) || dalloc(&n->v, o) || dalloc(&n->wb, o) ||
         dalloc(&n->Z1, bh) || dalloc(&n->H1, bh) || dalloc(&n->Z2, bh) || dalloc(&n->H2, bh) || dalloc(&n->Y, bo) ||
@@ -348,9 +345,9 @@ int qcml_set_params(qcml_net* n, const float* host)
     HIPCHK(hipMemcpy(n->w, host, n->nparam * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(n->m, 0, n->nparam * sizeof(float)));
     HIPCHK(hipMemset(n->v, 0, n->nparam * sizeof(float)));
-    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(n->nparam)), dim3(256), 0, n->stream, n->w, n->wb, n->nparam);
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(n->nparam)), dim3(256), 0, (hipStream_t)0, n->w, n->wb, n->nparam);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(n->stream));
+    HIPCHK(hipDeviceSynchronize());
     n->step = 0;
     return 0;
 }
@@ -359,14 +356,14 @@ int qcml_forward(qcml_net* n, const float* dX, int32_t B, float* dY, void* strea
 {
     if (!n || !dX || !dY || B < 1 || B > n->maxb) return 1;
     HIPCHK(hipSetDevice(n->device));
-    return forward(n, dX, B, dY, stream ? (hipStream_t)stream : n->stream);
+    return forward(n, dX, B, dY, (hipStream_t)stream);
 }
 
 int qcml_train_step(qcml_net* n, const float* dX, const float* dT, int32_t B, float lr, float* loss, void* stream)
 {
     if (!n || !dX || !dT || B < 1 || B > n->maxb) return 1;
     HIPCHK(hipSetDevice(n->device));
-    hipStream_t s = stream ? (hipStream_t)stream : n->stream;
+    hipStream_t s = (hipStream_t)stream;
     int nblk = 0;
     int rc = backward(n, dX, dT, B, s, &nblk);
     if (rc) return rc;
@@ -385,13 +382,13 @@ int qcml_gradients(qcml_net* n, const float* dX, const float* dT, int32_t B, flo
     if (!n || !dX || !dT || !host_grad || B < 1 || B > n->maxb) return 1;
     HIPCHK(hipSetDevice(n->device));
     int nblk = 0;
-    int rc = backward(n, dX, dT, B, n->stream, &nblk);
+    int rc = backward(n, dX, dT, B, (hipStream_t)0, &nblk);
     if (rc) return rc;
     if (loss) {
-        rc = read_loss(n, nblk, n->stream, loss, (long long)B * n->nout);
+        rc = read_loss(n, nblk, (hipStream_t)0, loss, (long long)B * n->nout);
         if (rc) return rc;
     }
-    HIPCHK(hipStreamSynchronize(n->stream));
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(host_grad, n->g, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }

@@ -122,7 +122,7 @@ class Net:
 
     def gradients(self, X, T):
         import torch
-        torch.cuda.current_stream().synchronize()  # inputs complete before the net's own stream reads them
+        torch.cuda.synchronize()  # inputs complete before the default-stream gradient pass reads them
         g = np.zeros(self.nparams, dtype=np.float32)
         loss = ctypes.c_float(0.0)
         rc = lib().qcml_gradients(self.h, _ptr(X), _ptr(T), X.shape[0], _f32p(g), ctypes.byref(loss))
