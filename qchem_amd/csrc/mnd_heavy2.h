@@ -5,11 +5,13 @@
 // over many primitive quartets:
 //   R batch        R^{(j)}_{tuv} for up to KB ket primitive pairs at once, level by level in LDS
 //                  (one barrier per level per batch, not per primitive quartet).
-//   ket transform  lane <-> bra Hermite index (t,u,v), lanes < nherm(Lab).  Each lane gathers its
-//                  window R_{t+tau,u+nu,v+phi} (tau+nu+phi <= Lcd) from LDS, then for every ket
-//                  component pair cd accumulates
+//   ket transform  lane <-> (bra Hermite index (t,u,v), group): G = 64 / nherm(Lab) groups of
+//                  nherm(Lab) lanes each take every G-th ket primitive pair of the batch.  Each lane
+//                  gathers its window R_{t+tau,u+nu,v+phi} (tau+nu+phi <= Lcd) from LDS, then for
+//                  every ket component pair cd accumulates
 //                  X[cd][tuv] += lam * sum (-1)^{tau+nu+phi} E^{cd}_{tau nu phi} R_{...}
-//                  with the ket Hermite tables in registers.
+//                  with the ket Hermite tables in registers; the groups' partial sums meet in LDS
+//                  (atomic adds).
 //   bra transform  lane <-> (bra primitive pair, ket component pair cd): gathers X[cd][*] of its
 //                  bra primitive pair from LDS and adds sum_tuv E^{ab}_{tuv} X[cd][tuv] into
 //                  I[ab][cd] (LDS FP64 atomics) for every ab.
@@ -25,6 +27,8 @@ struct H2Cfg {
     static constexpr int KB = S::NR <= 96 ? 9 : (S::NR <= 130 ? 8 : 6);  // ket prim pairs per R batch
     static constexpr int XI = S::NCD * S::NHAB;
     static constexpr int IB = XI >= 2304 ? 1 : (2304 / XI > 9 ? 9 : 2304 / XI);  // bra prim pairs per X batch
+    // ket transform: lane groups of NHAB lanes, each group on its own ket primitive pairs of the batch
+    static constexpr int G = 64 / S::NHAB > KB ? KB : (64 / S::NHAB < 1 ? 1 : 64 / S::NHAB);
 };
 
 // Per-wavefront LDS of the R batch and the ket-transformed block, in the quartet's arithmetic
@@ -68,9 +72,14 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
     constexpr int NHCD = nherm(S::LCD);
     for (int e = lane; e < S::NI; e += 64) sI[e] = 0.0;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;
-    const bool xl = lane < S::NHAB;
+    const bool xl = lane < C::G * S::NHAB;
+    const int grp = lane / S::NHAB, hl = lane - grp * S::NHAB;  // ket-transform group, Hermite index
     for (int ij0 = 0; ij0 < npab; ij0 += C::IB) {
         const int nib = npab - ij0 < C::IB ? npab - ij0 : C::IB;
+        if constexpr (C::G > 1) {
+            RT* X = &sm.X[0][0][0];
+            for (int e = lane; e < C::IB * S::NCD * S::NHAB; e += 64) X[e] = RT(0);  // seen after the next barrier
+        }
         for (int ii = 0; ii < nib; ++ii) {
             const double4 b0 = A.pp0[pi.z + ij0 + ii], b1 = A.pp1[pi.z + ij0 + ii];
             RT xacc[S::NCD];
@@ -116,7 +125,7 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     __syncthreads();
                 });
                 if (xl) {
-                    for (int s = 0; s < nb; ++s) {
+                    for (int s = grp; s < nb; s += C::G) {
                         const double4 k0 = A.pp0[qi.z + kl0 + s], k1 = A.pp1[qi.z + kl0 + s];
                         HermiteAxis<LC, LD, RT> ecd[3];
                         const RT c12 = RT(0.5 / k0.w);
@@ -162,7 +171,12 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     }
                 }
             }
-            if (xl) static_for<0, S::NCD>([&](auto k) { sm.X[ii][decltype(k)::value][lane] = xacc[decltype(k)::value]; });
+            if (xl) {
+                if constexpr (C::G > 1)
+                    static_for<0, S::NCD>([&](auto k) { atomicAdd(&sm.X[ii][decltype(k)::value][hl], xacc[decltype(k)::value]); });
+                else
+                    static_for<0, S::NCD>([&](auto k) { sm.X[ii][decltype(k)::value][hl] = xacc[decltype(k)::value]; });
+            }
         }
         __syncthreads();
         for (int pr = lane; pr < nib * S::NCD; pr += 64) {

@@ -446,7 +446,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
     const int lane = threadIdx.x;
     heavy2_init<LA, LB, LC, LD>(rtab, lane);
     int t0 = 0, u0 = 0, v0 = 0;
-    if (lane < S::NHAB) hdec(lane, t0, u0, v0);
+    if (lane < H2Cfg<LA, LB, LC, LD>::G * S::NHAB) hdec(lane % S::NHAB, t0, u0, v0);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
     UnitRange ur(C);
     for (long long u = ur.u; u < ur.end; ++u) {
