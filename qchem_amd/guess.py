@@ -91,7 +91,8 @@ class LearnedGuess:
     @classmethod
     def train(cls, X, Y, nbf, nocc, width=256, epochs=400, batch=128, lr=1e-3, seed=7, nval=64, log=print):
         """Adam with cosine decay on the MSE of the normalised orthogonalised density; the last nval
-        samples are held back as a validation set whose loss is logged."""
+        samples are held back as a validation set, checked every few epochs, and the parameters with
+        the lowest validation loss are kept (early stopping)."""
         import torch
         from qchem_amd import mlp
         Xv, Yv = X[-nval:], Y[-nval:]
@@ -113,6 +114,7 @@ class LearnedGuess:
         gen = torch.Generator(device="cpu").manual_seed(seed)
         steps = epochs * max(1, n // batch)
         t0, k = time.time(), 0
+        best, best_p, check = float("inf"), None, max(1, epochs // 40)
         for ep in range(epochs):
             perm = torch.randperm(n, generator=gen).cuda()
             for b0 in range(0, n - batch + 1, batch):
@@ -124,6 +126,12 @@ class LearnedGuess:
                     log("epoch %d batch loss %.5f validation loss %.5f (%.1f s)" % (ep, loss, val_loss(),
                                                                                    time.time() - t0))
                 k += 1
+            if ep % check == check - 1 or ep == epochs - 1:
+                v = val_loss()
+                if v < best:
+                    best, best_p = v, net.get_params()
+        net.set_params(best_p)
+        log("kept the parameters with validation loss %.5f" % best)
         return cls(net, x_mean, x_std, y_mean, y_scale, nbf, nocc)
 
     def predict_orth(self, mols):
