@@ -97,7 +97,7 @@ def scf_cpu_reference(mol, e_conv):
             "delta_e_vs_gpu": res["energy"] - e_conv}
 
 
-def scf_time_to_solution(qa, S, geometries=(100000, 100001, 100002, 100003), mixed_tol=1e-7):
+def scf_time_to_solution(qa, S, geometries=tuple(range(100000, 100008)), mixed_tol=1e-7):
     """Held-out C3H6O-like geometries: time to |E - E_conv| < 1e-6 Eh for the plain FP64 GPU path
     (SAD seed) and for mixed precision + the learned seed; E_conv from the FP64 run."""
     from qchem_amd import guess as G
