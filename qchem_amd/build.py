@@ -68,7 +68,9 @@ def build(verbose=False):
     host_objs = []
     for src in HOST_SOURCES:
         obj = os.path.join(OBJ, src.replace(".cpp", ".o"))
-        _run([cxx, "-std=c++20", "-O2", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + HOST, "-c",
+        # x86-64-v3 (AVX2 + FMA): every EPYC host of an MI355X has it; the dense host algebra of
+        # the SCF loop (products, tridiagonal eigensolver) runs ~3x faster with it
+        _run([cxx, "-std=c++20", "-O3", "-march=x86-64-v3", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + HOST, "-c",
               os.path.join(HOST, src), "-o", obj])
         host_objs.append(obj)
     hlib = os.path.join(HERE, "libqchem_host.so")

@@ -81,13 +81,25 @@ static void tridiagonalise(std::vector<double>& z, int n, std::vector<double>& d
                 e[i] = scale * g;
                 h -= f * g;
                 Z(i, l) = f - g;
+                // p = A u / h over the lower triangle, row by row (each row read once, contiguously)
                 f = 0.0;
                 for (int j = 0; j <= l; ++j) {
                     Z(j, i) = Z(i, j) / h;
-                    g = 0.0;
-                    for (int k = 0; k <= j; ++k) g += Z(j, k) * Z(i, k);
-                    for (int k = j + 1; k <= l; ++k) g += Z(k, j) * Z(i, k);
-                    e[j] = g / h;
+                    e[j] = 0.0;
+                }
+                const double* u = &Z(i, 0);
+                for (int j = 0; j <= l; ++j) {
+                    const double* row = &Z(j, 0);
+                    const double uj = u[j];
+                    double acc = row[j] * uj;
+                    for (int k = 0; k < j; ++k) {
+                        acc += row[k] * u[k];
+                        e[k] += row[k] * uj;
+                    }
+                    e[j] += acc;
+                }
+                for (int j = 0; j <= l; ++j) {
+                    e[j] /= h;
                     f += e[j] * Z(i, j);
                 }
                 const double hh = f / (h + h);
@@ -106,11 +118,19 @@ static void tridiagonalise(std::vector<double>& z, int n, std::vector<double>& d
     e[0] = 0.0;
     for (int i = 0; i < n; ++i) {
         const int l = i - 1;
-        if (d[i] != 0.0) {
-            for (int j = 0; j <= l; ++j) {
-                double g = 0.0;
-                for (int k = 0; k <= l; ++k) g += Z(i, k) * Z(k, j);
-                for (int k = 0; k <= l; ++k) Z(k, j) -= g * Z(k, i);
+        if (d[i] != 0.0 && l >= 0) {
+            // g_j = sum_k Z(i,k) Z(k,j), then Z(k,j) -= g_j Z(k,i): columns j are independent, so
+            // both passes run along rows
+            std::vector<double> g(l + 1, 0.0);
+            for (int k = 0; k <= l; ++k) {
+                const double zik = Z(i, k);
+                const double* row = &Z(k, 0);
+                for (int j = 0; j <= l; ++j) g[j] += zik * row[j];
+            }
+            for (int k = 0; k <= l; ++k) {
+                const double zki = Z(k, i);
+                double* row = &Z(k, 0);
+                for (int j = 0; j <= l; ++j) row[j] -= g[j] * zki;
             }
         }
         d[i] = Z(i, i);
@@ -119,10 +139,10 @@ static void tridiagonalise(std::vector<double>& z, int n, std::vector<double>& d
     }
 }
 
-// Implicit QL iteration on the tridiagonal (d, e); rotations accumulated into z's columns.
-static void ql_implicit(std::vector<double>& d, std::vector<double>& e, int n, std::vector<double>& z)
+// Implicit QL iteration on the tridiagonal (d, e); rotations accumulated into the ROWS of zt (the
+// transpose of the eigenvector matrix), so each rotation touches two contiguous rows.
+static void ql_implicit(std::vector<double>& d, std::vector<double>& e, int n, std::vector<double>& zt)
 {
-    auto Z = [&](int i, int j) -> double& { return z[(size_t)i * n + j]; };
     for (int i = 1; i < n; ++i) e[i - 1] = e[i];
     if (n > 0) e[n - 1] = 0.0;
     for (int l = 0; l < n; ++l) {
@@ -155,10 +175,12 @@ static void ql_implicit(std::vector<double>& d, std::vector<double>& e, int n, s
                     p = s * r;
                     d[i + 1] = g + p;
                     g = c * r - b;
+                    double* zi = &zt[(size_t)i * n];
+                    double* zj = &zt[(size_t)(i + 1) * n];
                     for (int k = 0; k < n; ++k) {
-                        f = Z(k, i + 1);
-                        Z(k, i + 1) = s * Z(k, i) + c * f;
-                        Z(k, i) = c * Z(k, i) - s * f;
+                        f = zj[k];
+                        zj[k] = s * zi[k] + c * f;
+                        zi[k] = c * zi[k] - s * f;
                     }
                 }
                 if (r == 0.0 && i >= l) continue;
@@ -182,7 +204,12 @@ void sym_eigen(const Mat& A, std::vector<double>& w, Mat& V)
         }
     if (n > 0) {
         tridiagonalise(z, n, d, e);
-        ql_implicit(d, e, n, z);
+        std::vector<double> zt((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) zt[(size_t)j * n + i] = z[(size_t)i * n + j];
+        ql_implicit(d, e, n, zt);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) z[(size_t)i * n + j] = zt[(size_t)j * n + i];
     }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
