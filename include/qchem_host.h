@@ -40,6 +40,9 @@ typedef struct qcscf_params {
     int32_t device;
     int32_t verbose;
     int32_t guess;     /* seed when no D0 is given: 0 = core Hamiltonian, 1 = SAD (atomic densities) */
+    int32_t incremental; /* > 0: J/K of the density change D_k - D_(k-1) added to the previous J/K (with
+                          * QCJK_SCREEN the change's small max|dD| screens most quartets late in the
+                          * SCF); a full build every `incremental` iterations.  0 = full builds. */
 } qcscf_params;
 
 typedef struct qcscf_result {

@@ -75,7 +75,8 @@ class _ScfParams(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("e_tol", ctypes.c_double), ("fd_tol", ctypes.c_double),
                 ("nproj", ctypes.c_int32), ("fd_max", ctypes.c_double), ("sv_tol", ctypes.c_double),
                 ("jk_flags", ctypes.c_uint32), ("screen_tol", ctypes.c_double), ("mixed_tol", ctypes.c_double),
-                ("device", ctypes.c_int32), ("verbose", ctypes.c_int32), ("guess", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("verbose", ctypes.c_int32), ("guess", ctypes.c_int32),
+                ("incremental", ctypes.c_int32)]
 
 GUESS_CORE, GUESS_SAD = 0, 1
 
@@ -369,12 +370,13 @@ class JKEngine:
 
 
 def rhf(basis, Z, R, charge=0, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_max=-1.0, sv_tol=5e-9,
-        jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, D0=None, verbose=False, guess=GUESS_CORE):
+        jk_flags=QCJK_J | QCJK_K, screen_tol=0.0, mixed_tol=0.0, device=0, D0=None, verbose=False, guess=GUESS_CORE,
+        incremental=0):
     """Closed-shell RHF through the C++ driver; returns a dict with energy, density and history."""
     Z = np.ascontiguousarray(Z, dtype=np.float64)
     R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
     par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device,
-                     1 if verbose else 0, int(guess))
+                     1 if verbose else 0, int(guess), int(incremental))
     res = _ScfResult()
     n = basis.nbf
     Dout = np.zeros((n, n))
@@ -418,7 +420,7 @@ def rhf_batch(bases, mols, max_iter=100, e_tol=1e-9, fd_tol=1e-6, nproj=4, fd_ma
     R = np.ascontiguousarray(np.concatenate([np.asarray(m["R"], dtype=np.float64).reshape(-1) for m in mols]))
     charge = np.array([int(m.get("charge", 0)) for m in mols], dtype=np.int32)
     par = _ScfParams(max_iter, e_tol, fd_tol, nproj, fd_max, sv_tol, jk_flags, screen_tol, mixed_tol, device,
-                     1 if verbose else 0, int(guess))
+                     1 if verbose else 0, int(guess), 0)
     d0 = None
     if D0 is not None:  # per-molecule seeds, packed back to back
         d0 = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in D0]))

@@ -58,8 +58,20 @@ struct ClassPairWork {
     int unit_list(int v) const { return (v == 0 || v == 3) ? 0 : 1; }
 };
 
-// Boys table F_j(0.1 k), k = 0..400, j = 0..23 (positive-term series, long double).
+// Boys table F_j(0.1 k), k = 0..400, j = 0..23 (positive-term series, long double), computed once
+// per process and shared by every handle.
+void make_boys_table_uncached(std::vector<double>& tab);
 void make_boys_table(std::vector<double>& tab)
+{
+    static const std::vector<double> cached = [] {
+        std::vector<double> t;
+        make_boys_table_uncached(t);
+        return t;
+    }();
+    tab = cached;
+}
+
+void make_boys_table_uncached(std::vector<double>& tab)
 {
     tab.assign(kBoysRows * kBoysCols, 0.0);
     for (int it = 0; it < kBoysRows; ++it) {

@@ -77,6 +77,7 @@ int qcscf_rhf(const qchost_shells* sh, int32_t natom, const double* Z, const dou
         p.sv_tol = par->sv_tol;
         p.verbose = par->verbose != 0;
         p.guess = par->guess;
+        p.incremental = par->incremental;
         Mat seed;
         if (D0) {
             seed = Mat(b.nbf);

@@ -201,7 +201,7 @@ Mat atomic_density(const Basis& ab, const Atom& atom, const qcjk_opts& o)
     sym_eigen(X.to_ortho(h), eps, Cp);
     Mat D = density_occ(X, Cp, averaged_occupations(eps, atom.Z));
     const WholeSystemFock_IBS<double>& whole = eng;
-    for (int it = 0; it < 200; ++it) {
+    for (int it = 0; it < 100; ++it) {
         const Mat J = whole.MakeAOFock(D, false);
         Mat F = axpby(1.0, h, 1.0, J);
         F = axpby(1.0, F, -0.5, whole.MakeAOFock(D, true));
@@ -209,7 +209,7 @@ Mat atomic_density(const Basis& ab, const Atom& atom, const qcjk_opts& o)
         const Mat Dn = density_occ(X, Cp, averaged_occupations(eps, atom.Z));
         const double change = frob(axpby(1.0, Dn, -1.0, D));
         D = axpby(0.5, Dn, 0.5, D);
-        if (change < 1e-7) break;
+        if (change < 1e-6) break;  // a seed: no need to go further
     }
     return D;
 }
@@ -376,12 +376,23 @@ SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const S
     // The Fock operator terms as the reference's Hamiltonian holds them: the two HF terms reach the
     // basis through the WholeSystemFock face (J first, then K, same density -> one fused build).
     const WholeSystemFock_IBS<double>& whole = eng;
+    Mat Dprev, Jprev, Kprev;  // incremental builds: J[D_k] = J[D_(k-1)] + J[D_k - D_(k-1)], same for K
     for (int it = 1; it <= par.max_iter; ++it) {
         const auto tj = clk::now();
+        const bool delta = par.incremental > 0 && it > 1 && (it - 1) % par.incremental != 0;
         Mat Dao(n);
-        whole.AddAODensity(Dao, run.D);
-        const Mat J = whole.MakeAOFock(Dao, false);
-        const Mat K = whole.MakeAOFock(Dao, true);
+        whole.AddAODensity(Dao, delta ? axpby(1.0, run.D, -1.0, Dprev) : run.D);
+        Mat J = whole.MakeAOFock(Dao, false);
+        Mat K = whole.MakeAOFock(Dao, true);
+        if (delta) {
+            J += Jprev;
+            K += Kprev;
+        }
+        if (par.incremental > 0) {
+            Dprev = run.D;
+            Jprev = J;
+            Kprev = K;
+        }
         run.res.t_jk += secs(tj, clk::now());
         Mat F = run.h;
         whole.SliceAOFock(F, J);
@@ -459,11 +470,17 @@ BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vect
     if (!seeds && par.guess == 1) {
         qcjk_opts ao{};
         ao.device = opts.device;
-        for (int m = 0; m < nmol; ++m) sad[m] = sad_density(*bases[m], atoms[m], ao);
+        for (int m = 0; m < nmol; ++m) {
+            sad[m] = sad_density(*bases[m], atoms[m], ao);
+            if (par.verbose && (m + 1) % 32 == 0)
+                std::fprintf(stderr, "batch SAD seeds: %d/%d molecules, %.2f s\n", m + 1, nmol, secs(t_start, clk::now()));
+        }
     }
+    SCFParams quiet = par;  // per-molecule iteration lines would be nmol per iteration
+    quiet.verbose = false;
     parallel([&](int m) {
         const Mat* d0 = seeds ? &(*seeds)[m] : (par.guess == 1 ? &sad[m] : nullptr);
-        runs[m].setup(S[m], h1[m], atoms[m], nelec[m], par, d0);
+        runs[m].setup(S[m], h1[m], atoms[m], nelec[m], quiet, d0);
     });
     out.t_setup = secs(t_start, clk::now());
 
@@ -496,9 +513,12 @@ BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vect
             r.after_build(it, F, J, K, t_now);
         });
         out.t_host += secs(th, clk::now());
-        bool all = true;
-        for (const RHFRun& r : runs) all = all && r.done;
-        if (all) break;
+        int ndone = 0;
+        for (const RHFRun& r : runs) ndone += r.done ? 1 : 0;
+        if (par.verbose)  // one progress line per batched iteration
+            std::fprintf(stderr, "batch iteration %d: %d/%d converged, %.2f s\n", it, ndone, nmol,
+                         secs(t_start, clk::now()));
+        if (ndone == nmol) break;
     }
     out.t_total = secs(t_start, clk::now());
     out.mols.reserve(nmol);

@@ -36,6 +36,7 @@ struct SCFParams {
     double sv_tol = 5e-9;    // DIISParams SVTol
     bool verbose = false;
     int guess = 0;           // 0 core Hamiltonian, 1 SAD (used when no explicit seed is given)
+    int incremental = 0;     // > 0: J/K from the density change, full rebuild every `incremental` iterations
 };
 
 struct SCFResult {

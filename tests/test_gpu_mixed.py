@@ -107,3 +107,19 @@ def test_mixed_rhf_energy(require_gpu):
     assert r64["converged"] and rmx["converged"]
     assert rmx["fp32_fraction"] > 0.05
     assert abs(rmx["energy"] - r64["energy"]) < 1e-6
+
+
+def test_incremental_screened_rhf(require_gpu):
+    """Incremental builds (J/K of the density change, Schwarz-screened at 1e-12, full rebuild every
+    8 iterations) converge to the full-build energy."""
+    m = S.acetone_like(3)
+    B = qa.Basis(m["shells"])
+    full = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7, guess=qa.GUESS_SAD)
+    inc = qa.rhf(B, m["Z"], m["R"], e_tol=1e-10, fd_tol=1e-7, guess=qa.GUESS_SAD, incremental=8,
+                 jk_flags=qa.QCJK_J | qa.QCJK_K | qa.QCJK_SCREEN, screen_tol=1e-12)
+    print("full %.10f (%d it, %.3f s jk)  incremental %.10f (%d it, %.3f s jk, %d quartets vs %d)" % (
+        full["energy"], full["iterations"], full["t_jk"], inc["energy"], inc["iterations"], inc["t_jk"],
+        inc["quartets"], full["quartets"]))
+    assert full["converged"] and inc["converged"]
+    assert abs(inc["energy"] - full["energy"]) < 1e-8
+    assert inc["quartets"] < full["quartets"]
