@@ -187,78 +187,175 @@ std::vector<double> averaged_occupations(const std::vector<double>& eps, double 
     return occ;
 }
 
-// Spherically averaged neutral-atom density of one atom in its own shells: fractional-occupation
-// RHF (averaged open shell), damped fixed point.
-Mat atomic_density(const Basis& ab, const Atom& atom, const qcjk_opts& o)
+// Run fn(m) for m in [0, n) on a small pool of threads (the molecules or atoms are independent).
+template <typename Fn>
+void parallel_for(int n, int nthreads, Fn&& fn)
 {
-    GpuHF_IBS eng(ab.view(), o);
-    Mat S, h;
-    one_electron(eng, {atom}, S, h);
-    Ortho X;
-    X.set(S);
-    std::vector<double> eps;
-    Mat Cp;
-    sym_eigen(X.to_ortho(h), eps, Cp);
-    Mat D = density_occ(X, Cp, averaged_occupations(eps, atom.Z));
-    const WholeSystemFock_IBS<double>& whole = eng;
+    nthreads = std::max(1, std::min(nthreads, n));
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> errs(nthreads);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&, t] {
+            try {
+                for (int m = next++; m < n; m = next++) fn(m);
+            } catch (...) {
+                errs[t] = std::current_exception();
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
+int host_threads() { return (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+// Spherically averaged neutral-atom densities, each atom in its own shells: fractional-occupation
+// RHF (averaged open shell), damped fixed point.  All atoms share one batch handle, so every
+// iteration is one batched J/K build for all of them.
+std::vector<Mat> atomic_densities(const std::vector<Basis>& ab, const std::vector<Atom>& at, const qcjk_opts& o)
+{
+    const int na = (int)ab.size();
+    std::vector<Mat> D(na);
+    if (na == 0) return D;
+    std::vector<qcjk_basis> views(na);
+    for (int m = 0; m < na; ++m) views[m] = ab[m].view();
+    qcjk_handle* h = nullptr;
+    int rc = qcjk_create_batch(views.data(), na, &o, &h);
+    if (rc != QCJK_OK) throw EngineError("qcjk_create_batch", rc);
+    struct Guard {
+        qcjk_handle* h;
+        ~Guard() { qcjk_destroy(h); }
+    } guard{h};
+    std::vector<int32_t> nbf(na);
+    std::vector<int64_t> off(na);
+    int64_t nelem = 0;
+    rc = qcjk_layout(h, nullptr, nbf.data(), off.data(), &nelem);
+    if (rc != QCJK_OK) throw EngineError("qcjk_layout", rc);
+    std::vector<Mat> hcore(na);
+    std::vector<Ortho> X(na);
+    for (int m = 0; m < na; ++m) {
+        const int n = nbf[m];
+        Mat S(n), T(n), V(n);
+        rc = qcjk_one_electron_mol(h, m, 1, &at[m].Z, at[m].R, S.a.data(), T.a.data(), V.a.data());
+        if (rc != QCJK_OK) throw EngineError("qcjk_one_electron_mol", rc);
+        hcore[m] = axpby(1.0, T, 1.0, V);
+        X[m].set(S);
+    }
+    parallel_for(na, host_threads(), [&](int m) {
+        std::vector<double> eps;
+        Mat Cp;
+        sym_eigen(X[m].to_ortho(hcore[m]), eps, Cp);
+        D[m] = density_occ(X[m], Cp, averaged_occupations(eps, at[m].Z));
+    });
+    std::vector<char> done(na, 0);
+    std::vector<double> Dp(nelem), Jp(nelem), Kp(nelem);
     for (int it = 0; it < 100; ++it) {
-        const Mat J = whole.MakeAOFock(D, false);
-        Mat F = axpby(1.0, h, 1.0, J);
-        F = axpby(1.0, F, -0.5, whole.MakeAOFock(D, true));
-        sym_eigen(X.to_ortho(F), eps, Cp);
-        const Mat Dn = density_occ(X, Cp, averaged_occupations(eps, atom.Z));
-        const double change = frob(axpby(1.0, Dn, -1.0, D));
-        D = axpby(0.5, Dn, 0.5, D);
-        if (change < 1e-6) break;  // a seed: no need to go further
+        for (int m = 0; m < na; ++m) std::memcpy(Dp.data() + off[m], D[m].a.data(), D[m].a.size() * sizeof(double));
+        rc = qcjk_build(h, Dp.data(), Jp.data(), Kp.data(), QCJK_J | QCJK_K | QCJK_GRAPH, nullptr);
+        if (rc != QCJK_OK) throw EngineError("qcjk_build", rc);
+        parallel_for(na, host_threads(), [&](int m) {
+            if (done[m]) return;
+            const int n = nbf[m];
+            Mat F = hcore[m];
+            for (int k = 0; k < n * n; ++k) F.a[k] += Jp[off[m] + k] - 0.5 * Kp[off[m] + k];
+            std::vector<double> eps;
+            Mat Cp;
+            sym_eigen(X[m].to_ortho(F), eps, Cp);
+            const Mat Dn = density_occ(X[m], Cp, averaged_occupations(eps, at[m].Z));
+            const double change = frob(axpby(1.0, Dn, -1.0, D[m]));
+            D[m] = axpby(0.5, Dn, 0.5, D[m]);
+            if (change < 1e-6) done[m] = 1;  // a seed: no need to go further
+        });
+        bool all = true;
+        for (char c : done) all = all && c;
+        if (all) break;
     }
     return D;
 }
 
 }  // namespace
 
-Mat sad_density(const Basis& b, const std::vector<Atom>& atoms, const qcjk_opts& o)
+std::vector<Mat> sad_densities(const std::vector<const Basis*>& bases, const std::vector<std::vector<Atom>>& atoms,
+                               const qcjk_opts& o)
 {
-    Mat D(b.nbf);
-    std::vector<int> bf0(b.shells.size() + 1, 0);
-    for (size_t s = 0; s < b.shells.size(); ++s) bf0[s + 1] = bf0[s] + (b.shells[s].L + 1) * (b.shells[s].L + 2) / 2;
     // (atom signature, density), kept for the process: an element with the same shells always gets
     // the same atomic density, as from a tabulated SAD database
     static std::mutex cache_mu;
     static std::vector<std::pair<std::vector<double>, Mat>> cache;
-    for (const Atom& a : atoms) {
-        Basis ab;
-        std::vector<int> idx;  // molecular basis-function index of each atomic one
-        for (size_t s = 0; s < b.shells.size(); ++s) {
-            const Shell& sh = b.shells[s];
-            const double dx = sh.R[0] - a.R[0], dy = sh.R[1] - a.R[1], dz = sh.R[2] - a.R[2];
-            if (dx * dx + dy * dy + dz * dz > 1e-16) continue;
-            ab.shells.push_back(sh);
-            for (int k = bf0[s]; k < bf0[s + 1]; ++k) idx.push_back(k);
-        }
-        if (ab.shells.empty()) continue;
-        // same element with the same shells (translation-invariant): reuse the atomic density
-        std::vector<double> sig{a.Z};
-        for (const Shell& sh : ab.shells) {
-            sig.push_back(sh.L + 100.0 * sh.contracted);
-            sig.insert(sig.end(), sh.exps.begin(), sh.exps.end());
-            sig.insert(sig.end(), sh.coefs.begin(), sh.coefs.end());
-        }
+    struct Place {
+        int mol;
+        std::vector<int> idx;      // molecular basis-function index of each atomic one
+        std::vector<double> sig;
+        int todo = -1;             // index into the batch of atoms still to compute
         Mat Da;
-        {
-            std::lock_guard<std::mutex> lk(cache_mu);
-            for (const auto& c : cache)
-                if (c.first == sig) Da = c.second;
+    };
+    std::vector<Place> places;
+    std::vector<Basis> todo_basis;
+    std::vector<Atom> todo_atom;
+    std::vector<std::vector<double>> todo_sig;
+    for (size_t mi = 0; mi < bases.size(); ++mi) {
+        const Basis& b = *bases[mi];
+        std::vector<int> bf0(b.shells.size() + 1, 0);
+        for (size_t s = 0; s < b.shells.size(); ++s) bf0[s + 1] = bf0[s] + (b.shells[s].L + 1) * (b.shells[s].L + 2) / 2;
+        for (const Atom& a : atoms[mi]) {
+            Place pl;
+            pl.mol = (int)mi;
+            Basis ab;
+            for (size_t s = 0; s < b.shells.size(); ++s) {
+                const Shell& sh = b.shells[s];
+                const double dx = sh.R[0] - a.R[0], dy = sh.R[1] - a.R[1], dz = sh.R[2] - a.R[2];
+                if (dx * dx + dy * dy + dz * dz > 1e-16) continue;
+                ab.shells.push_back(sh);
+                for (int k = bf0[s]; k < bf0[s + 1]; ++k) pl.idx.push_back(k);
+            }
+            if (ab.shells.empty()) continue;
+            // same element with the same shells (translation-invariant): one atomic density
+            pl.sig = {a.Z};
+            for (const Shell& sh : ab.shells) {
+                pl.sig.push_back(sh.L + 100.0 * sh.contracted);
+                pl.sig.insert(pl.sig.end(), sh.exps.begin(), sh.exps.end());
+                pl.sig.insert(pl.sig.end(), sh.coefs.begin(), sh.coefs.end());
+            }
+            {
+                std::lock_guard<std::mutex> lk(cache_mu);
+                for (const auto& c : cache)
+                    if (c.first == pl.sig) pl.Da = c.second;
+            }
+            if (pl.Da.n == 0) {
+                for (size_t t = 0; t < todo_sig.size() && pl.todo < 0; ++t)
+                    if (todo_sig[t] == pl.sig) pl.todo = (int)t;
+                if (pl.todo < 0) {
+                    pl.todo = (int)todo_sig.size();
+                    ab.finalize();
+                    Atom centred = a;
+                    todo_basis.push_back(std::move(ab));
+                    todo_atom.push_back(centred);
+                    todo_sig.push_back(pl.sig);
+                }
+            }
+            places.push_back(std::move(pl));
         }
-        if (Da.n == 0) {
-            ab.finalize();
-            Da = atomic_density(ab, a, o);
-            std::lock_guard<std::mutex> lk(cache_mu);
-            cache.emplace_back(sig, Da);
-        }
-        for (size_t i = 0; i < idx.size(); ++i)
-            for (size_t j = 0; j < idx.size(); ++j) D(idx[i], idx[j]) = Da((int)i, (int)j);
     }
-    return D;
+    const std::vector<Mat> fresh = atomic_densities(todo_basis, todo_atom, o);
+    {
+        std::lock_guard<std::mutex> lk(cache_mu);
+        for (size_t t = 0; t < fresh.size(); ++t) cache.emplace_back(todo_sig[t], fresh[t]);
+    }
+    std::vector<Mat> out;
+    for (const Basis* b : bases) out.emplace_back(b->nbf);
+    for (Place& pl : places) {
+        const Mat& Da = pl.todo >= 0 ? fresh[pl.todo] : pl.Da;
+        Mat& D = out[pl.mol];
+        for (size_t i = 0; i < pl.idx.size(); ++i)
+            for (size_t j = 0; j < pl.idx.size(); ++j) D(pl.idx[i], pl.idx[j]) = Da((int)i, (int)j);
+    }
+    return out;
+}
+
+Mat sad_density(const Basis& b, const std::vector<Atom>& atoms, const qcjk_opts& o)
+{
+    return sad_densities({&b}, {atoms}, o)[0];
 }
 
 double nuclear_repulsion(const std::vector<Atom>& atoms)
@@ -433,23 +530,7 @@ BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vect
     if (rc != QCJK_OK) throw EngineError("qcjk_layout", rc);
 
     // per-molecule work on a small thread pool (molecules are independent)
-    nthreads = std::max(1, std::min(nthreads, nmol));
-    auto parallel = [&](auto&& fn) {
-        std::atomic<int> next{0};
-        std::vector<std::exception_ptr> errs(nthreads);
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nthreads; ++t)
-            pool.emplace_back([&, t] {
-                try {
-                    for (int m = next++; m < nmol; m = next++) fn(m);
-                } catch (...) {
-                    errs[t] = std::current_exception();
-                }
-            });
-        for (auto& th : pool) th.join();
-        for (auto& e : errs)
-            if (e) std::rethrow_exception(e);
-    };
+    auto parallel = [&](auto&& fn) { parallel_for(nmol, nthreads, fn); };
 
     std::vector<RHFRun> runs(nmol);
     std::vector<Mat> S(nmol), h1(nmol);
@@ -470,11 +551,8 @@ BatchSCFResult rhf_batch(const std::vector<const Basis*>& bases, const std::vect
     if (!seeds && par.guess == 1) {
         qcjk_opts ao{};
         ao.device = opts.device;
-        for (int m = 0; m < nmol; ++m) {
-            sad[m] = sad_density(*bases[m], atoms[m], ao);
-            if (par.verbose && (m + 1) % 32 == 0)
-                std::fprintf(stderr, "batch SAD seeds: %d/%d molecules, %.2f s\n", m + 1, nmol, secs(t_start, clk::now()));
-        }
+        sad = sad_densities(bases, atoms, ao);
+        if (par.verbose) std::fprintf(stderr, "batch SAD seeds: %d molecules, %.2f s\n", nmol, secs(t_start, clk::now()));
     }
     SCFParams quiet = par;  // per-molecule iteration lines would be nmol per iteration
     quiet.verbose = false;

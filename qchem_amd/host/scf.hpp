@@ -61,6 +61,9 @@ void one_electron(GpuHF_IBS& eng, const std::vector<Atom>& atoms, Mat& S, Mat& h
 // matrix form of the seed; the reference's SAD (ChargeDensity/AtomicDensity.C) reads tabulated
 // radial densities instead.  Atoms are matched to shells by centre.
 Mat sad_density(const Basis& b, const std::vector<Atom>& atoms, const qcjk_opts& o);
+// The same for many molecules at once; the atomic SCFs not yet cached run as one batch.
+std::vector<Mat> sad_densities(const std::vector<const Basis*>& bases, const std::vector<std::vector<Atom>>& atoms,
+                               const qcjk_opts& o);
 
 // RHF to convergence.  D0 (optional) is an explicit seed density; otherwise the core guess.
 SCFResult rhf(GpuHF_IBS& eng, const std::vector<Atom>& atoms, int nelec, const SCFParams& par, const Mat* D0 = nullptr,
