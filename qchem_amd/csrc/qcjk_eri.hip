@@ -9,11 +9,10 @@ template <int LA, int LB, int LC, int LD>
 constexpr ClassKernelInfo make_info()
 {
     static_assert(has_light(LA, LB, LC, LD) == light_listed(LA, LB, LC, LD), "lane-kernel class list out of date");
-    static_assert((has_light(LA, LB, LC, LD) && LC > 0) == lightc_listed(LA, LB, LC, LD),
-                  "chunked lane-kernel class list out of date");
+    static_assert(has_lightc(LA, LB, LC, LD) == lightc_listed(LA, LB, LC, LD), "chunked lane-kernel class list out of date");
     ClassLauncher light = nullptr, lightc = nullptr;
     if constexpr (has_light(LA, LB, LC, LD)) light = &launch_light<LA, LB, LC, LD>;
-    if constexpr (has_light(LA, LB, LC, LD) && LC > 0) lightc = &launch_lightc<LA, LB, LC, LD>;
+    if constexpr (has_lightc(LA, LB, LC, LD)) lightc = &launch_lightc<LA, LB, LC, LD>;
     return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1,
             default_light(LA, LB, LC, LD) ? light : &launch_heavy2<LA, LB, LC, LD>, light,
             &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>, lightc};

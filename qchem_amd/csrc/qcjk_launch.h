@@ -21,6 +21,13 @@ constexpr bool default_light(int la, int lb, int lc, int ld)
 {
     return has_light(la, lb, lc, ld) && !(la == 2 && lb == 2);
 }
+// Classes with the chunked lane-per-quartet variant: the lane-kernel classes with a p or d ket, and
+// (dp|ds), whose per-chunk state still fits a lane without scratch ((dp|pp), (dd|ps) and (dd|pp)
+// spill 1-5 KB per lane to scratch in FP64 and are not compiled).
+constexpr bool has_lightc(int la, int lb, int lc, int ld)
+{
+    return (has_light(la, lb, lc, ld) && lc > 0) || (la == 2 && lb == 1 && lc == 2 && ld == 0);
+}
 // Bra Hermite tables go to LDS once they would cost more registers than the rest of the lane's state.
 constexpr bool bra_in_lds(int la, int lb) { return la + lb >= 3; }
 
@@ -33,7 +40,8 @@ constexpr bool bra_in_lds(int la, int lb) { return la + lb >= 3; }
     X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 1, 1) X(2, 0, 0, 0)        \
     X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 2, 0, 0)
 #define QCJK_LIGHTC_CLASSES(X)                                                                           \
-    X(1, 0, 1, 0) X(1, 1, 1, 0) X(1, 1, 1, 1) X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 1, 0)
+    X(1, 0, 1, 0) X(1, 1, 1, 0) X(1, 1, 1, 1) X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 1, 0)        \
+    X(2, 1, 2, 0)
 
 #define QCJK_CLASS_MATCH(a, b, c, d) || (la == a && lb == b && lc == c && ld == d)
 constexpr bool light_listed(int la, int lb, int lc, int ld) { return false QCJK_LIGHT_CLASSES(QCJK_CLASS_MATCH); }
