@@ -40,6 +40,10 @@ constexpr int comp(int l, int k, int axis)
     return 0;
 }
 
+// Triangular index of (u, v), u + v <= M: by s = u + v, then v, for arrays sized nherm2(M).
+constexpr int nherm2(int m) { return (m + 1) * (m + 2) / 2; }
+constexpr int hidx2(int u, int v) { return (u + v) * (u + v + 1) / 2 + v; }
+
 // Triangular Hermite index of (t,u,v), t+u+v = s: shells by s, then t descending, then v.
 constexpr int hidx(int t, int u, int v)
 {

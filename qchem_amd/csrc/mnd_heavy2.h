@@ -14,7 +14,10 @@
 //                  (atomic adds).
 //   bra transform  lane <-> (bra primitive pair, ket component pair cd): gathers X[cd][*] of its
 //                  bra primitive pair from LDS and adds sum_tuv E^{ab}_{tuv} X[cd][tuv] into
-//                  I[ab][cd] (LDS FP64 atomics) for every ab.
+//                  I[ab][cd] (LDS FP64 atomics) for every ab: with the bra Hermite tables in
+//                  registers, or (FACT) one axis at a time (Y[u][v] = sum_t Ex X[t][u][v],
+//                  Z[v] = sum_u Ey Y[u][v], I = sum_v Ez Z[v]) with the tables of the batch's bra
+//                  primitive pairs in LDS: fewer registers and FLOPs, more LDS.
 #pragma once
 
 #include "mnd_heavy.h"
@@ -33,7 +36,7 @@ struct H2Cfg {
 
 // Per-wavefront LDS of the R batch and the ket-transformed block, in the quartet's arithmetic
 // type.  The R recursion table and the contracted block I (always FP64) live outside it.
-template <int LA, int LB, int LC, int LD, typename RT>
+template <int LA, int LB, int LC, int LD, typename RT, bool FACT = false>
 struct H2Smem {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD>;
@@ -41,7 +44,14 @@ struct H2Smem {
     RT top[C::KB][S::L + 1];  // (-2 alpha)^j F_j per batch slot
     RT geo[C::KB][4];         // P-Q (x,y,z), lambda per batch slot
     RT X[C::IB][S::NCD][S::NHAB];
+    HermiteAxis<LA, LB, RT> E[FACT ? C::IB : 1][3];  // FACT: bra Hermite tables of a batch's bra primitive pairs
 };
+
+// Reference-order index of the Cartesian component (x, y, z) of a shell of angular momentum l.
+constexpr int cidx(int l, int y, int z) { return z * (l + 1) - z * (z - 1) / 2 + y; }
+static_assert(cidx(2, 0, 1) == 3 && comp(2, 3, 0) == 1 && comp(2, 3, 2) == 1 && cidx(2, 0, 2) == 5 &&
+                  cidx(1, 1, 0) == 1 && comp(1, 1, 1) == 1,
+              "component index must invert comp()");
 
 // Build the per-entry R recursion table (once per kernel): dir | i1<<2 | i2<<10 | c2<<18.
 template <int LA, int LB, int LC, int LD>
@@ -62,8 +72,8 @@ __device__ __forceinline__ void heavy2_init(int* rtab, int lane)
 
 // One contracted quartet into sI (FP64, NAB x NCD).  RT = float evaluates the primitive integrals
 // in FP32 (the Boys function at T >= 40 and the (-2 alpha)^j F_j and lambda products in FP64).
-template <int LA, int LB, int LC, int LD, typename RT>
-__device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD, RT>& sm,
+template <int LA, int LB, int LC, int LD, typename RT, bool FACT>
+__device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD, RT, FACT>& sm,
                                                const int* __restrict__ rtab, double* sI, int lane, int t0, int u0,
                                                int v0, int4 pi, double4 pg, int4 qi, double4 qg)
 {
@@ -178,38 +188,99 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     static_for<0, S::NCD>([&](auto k) { sm.X[ii][decltype(k)::value][hl] = xacc[decltype(k)::value]; });
             }
         }
-        __syncthreads();
-        for (int pr = lane; pr < nib * S::NCD; pr += 64) {
-            const int ii = pr / S::NCD, cd = pr - ii * S::NCD;
-            const double4 b0 = A.pp0[pi.z + ij0 + ii], b1 = A.pp1[pi.z + ij0 + ii];
-            HermiteAxis<LA, LB, RT> eab[3];
-            const RT a12 = RT(0.5 / b0.w);
-            eab[0].build(a12, RT(b1.x), RT(b1.x + pg.x));
-            eab[1].build(a12, RT(b1.y), RT(b1.y + pg.y));
-            eab[2].build(a12, RT(b1.z), RT(b1.z + pg.z));
-            RT xw[S::NHAB];
-            static_for<0, S::NHAB>([&](auto h) { xw[decltype(h)::value] = sm.X[ii][cd][decltype(h)::value]; });
-            static_for<0, S::NA>([&](auto aa) {
-                constexpr int a = decltype(aa)::value;
-                constexpr int ax = comp(LA, a, 0), ay = comp(LA, a, 1), az = comp(LA, a, 2);
-                static_for<0, S::NB>([&](auto bb) {
-                    constexpr int b = decltype(bb)::value;
-                    constexpr int bx = comp(LB, b, 0), by = comp(LB, b, 1), bz = comp(LB, b, 2);
-                    RT acc = RT(0);
-                    static_for<0, ax + bx + 1>([&](auto tt) {
-                        constexpr int t = decltype(tt)::value;
-                        static_for<0, ay + by + 1>([&](auto uu) {
+        if constexpr (!FACT) {
+            __syncthreads();
+            for (int pr = lane; pr < nib * S::NCD; pr += 64) {
+                const int ii = pr / S::NCD, cd = pr - ii * S::NCD;
+                const double4 b0 = A.pp0[pi.z + ij0 + ii], b1 = A.pp1[pi.z + ij0 + ii];
+                HermiteAxis<LA, LB, RT> eab[3];
+                const RT a12 = RT(0.5 / b0.w);
+                eab[0].build(a12, RT(b1.x), RT(b1.x + pg.x));
+                eab[1].build(a12, RT(b1.y), RT(b1.y + pg.y));
+                eab[2].build(a12, RT(b1.z), RT(b1.z + pg.z));
+                RT xw[S::NHAB];
+                static_for<0, S::NHAB>([&](auto h) { xw[decltype(h)::value] = sm.X[ii][cd][decltype(h)::value]; });
+                static_for<0, S::NA>([&](auto aa) {
+                    constexpr int a = decltype(aa)::value;
+                    constexpr int ax = comp(LA, a, 0), ay = comp(LA, a, 1), az = comp(LA, a, 2);
+                    static_for<0, S::NB>([&](auto bb) {
+                        constexpr int b = decltype(bb)::value;
+                        constexpr int bx = comp(LB, b, 0), by = comp(LB, b, 1), bz = comp(LB, b, 2);
+                        RT acc = RT(0);
+                        static_for<0, ax + bx + 1>([&](auto tt) {
+                            constexpr int t = decltype(tt)::value;
+                            static_for<0, ay + by + 1>([&](auto uu) {
+                                constexpr int u = decltype(uu)::value;
+                                const RT exy = eab[0].e[ax][bx][t] * eab[1].e[ay][by][u];
+                                static_for<0, az + bz + 1>([&](auto vv) {
+                                    constexpr int v = decltype(vv)::value;
+                                    acc += exy * eab[2].e[az][bz][v] * xw[hidx(t, u, v)];
+                                });
+                            });
+                        });
+                        atomicAdd(&sI[(a * S::NB + b) * S::NCD + cd], (double)acc);
+                    });
+                });
+            }
+        } else {
+            for (int tk = lane; tk < nib * 3; tk += 64) {  // bra Hermite tables of the batch, one axis per lane
+                const int ii = tk / 3, ax = tk - ii * 3;
+                const double4 b0 = A.pp0[pi.z + ij0 + ii], b1 = A.pp1[pi.z + ij0 + ii];
+                const double pa = ax == 0 ? b1.x : (ax == 1 ? b1.y : b1.z);
+                const double ab = ax == 0 ? pg.x : (ax == 1 ? pg.y : pg.z);
+                HermiteAxis<LA, LB, RT> e;
+                e.build(RT(0.5 / b0.w), RT(pa), RT(pa + ab));
+                e.store(sm.E[ii][ax]);
+            }
+            __syncthreads();
+            for (int pr = lane; pr < nib * S::NCD; pr += 64) {
+                const int ii = pr / S::NCD, cd = pr - ii * S::NCD;
+                const HermiteAxis<LA, LB, RT>& Ex = sm.E[ii][0];
+                const HermiteAxis<LA, LB, RT>& Ey = sm.E[ii][1];
+                const HermiteAxis<LA, LB, RT>& Ez = sm.E[ii][2];
+                RT xw[S::NHAB];
+                static_for<0, S::NHAB>([&](auto h) { xw[decltype(h)::value] = sm.X[ii][cd][decltype(h)::value]; });
+                static_for<0, LA + 1>([&](auto axx) {
+                    constexpr int ax = decltype(axx)::value;
+                    static_for<0, LB + 1>([&](auto bxx) {
+                        constexpr int bx = decltype(bxx)::value;
+                        constexpr int M = (LA - ax) + (LB - bx);  // u + v <= M
+                        // Y[u][v] = sum_t Ex[ax][bx][t] X[t][u][v]
+                        RT Y[nherm2(M)];
+                        static_for<0, M + 1>([&](auto uu) {
                             constexpr int u = decltype(uu)::value;
-                            const RT exy = eab[0].e[ax][bx][t] * eab[1].e[ay][by][u];
-                            static_for<0, az + bz + 1>([&](auto vv) {
+                            static_for<0, M + 1 - u>([&](auto vv) {
                                 constexpr int v = decltype(vv)::value;
-                                acc += exy * eab[2].e[az][bz][v] * xw[hidx(t, u, v)];
+                                RT y = RT(0);
+                                static_for<0, ax + bx + 1>([&](auto tt) {
+                                    constexpr int t = decltype(tt)::value;
+                                    y += Ex.e[ax][bx][t] * xw[hidx(t, u, v)];
+                                });
+                                Y[hidx2(u, v)] = y;
+                            });
+                        });
+                        static_for<0, LA - ax + 1>([&](auto ayy) {
+                            constexpr int ay = decltype(ayy)::value, az = LA - ax - ay;
+                            static_for<0, LB - bx + 1>([&](auto byy) {
+                                constexpr int by = decltype(byy)::value, bz = LB - bx - by;
+                                // Z[v] = sum_u Ey[ay][by][u] Y[u][v]; I = sum_v Ez[az][bz][v] Z[v]
+                                RT acc = RT(0);
+                                static_for<0, az + bz + 1>([&](auto vv) {
+                                    constexpr int v = decltype(vv)::value;
+                                    RT z = RT(0);
+                                    static_for<0, ay + by + 1>([&](auto uu) {
+                                        constexpr int u = decltype(uu)::value;
+                                        z += Ey.e[ay][by][u] * Y[hidx2(u, v)];
+                                    });
+                                    acc += Ez.e[az][bz][v] * z;
+                                });
+                                constexpr int a = cidx(LA, ay, az), b = cidx(LB, by, bz);
+                                atomicAdd(&sI[(a * S::NB + b) * S::NCD + cd], (double)acc);
                             });
                         });
                     });
-                    atomicAdd(&sI[(a * S::NB + b) * S::NCD + cd], (double)acc);
                 });
-            });
+            }
         }
         __syncthreads();
     }
