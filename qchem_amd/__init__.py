@@ -345,7 +345,7 @@ class JKEngine:
             raise EngineError("qcjk_set_variant", rc)
 
     def autotune(self, dD, reps=3):
-        """Time both kernel variants per class on device density dD; keep the faster (see tune.py)."""
+        """Time every kernel variant per class on device density dD; keep the fastest (see tune.py)."""
         cnt = ctypes.c_int(0)
         arr = (ClassStat * 64)()
         ptr = dD if isinstance(dD, int) else dD.data_ptr()

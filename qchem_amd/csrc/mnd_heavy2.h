@@ -24,22 +24,27 @@
 
 namespace qcjk {
 
-template <int LA, int LB, int LC, int LD>
+// Batch sizes.  SMALL trades batching for occupancy: about 8 KB each for the R batch and the X
+// batch, so a one-wavefront block fits ~20 KB of LDS and two blocks run per SIMD.
+template <int LA, int LB, int LC, int LD, bool SMALL = false>
 struct H2Cfg {
     using S = HeavyShape<LA, LB, LC, LD>;
-    static constexpr int KB = S::NR <= 96 ? 9 : (S::NR <= 130 ? 8 : 6);  // ket prim pairs per R batch
+    static constexpr int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+    static constexpr int KB0 = S::NR <= 96 ? 9 : (S::NR <= 130 ? 8 : 6);
+    static constexpr int KB = SMALL ? clampi(8192 / (16 * S::NR), 3, KB0) : KB0;  // ket prim pairs per R batch
     static constexpr int XI = S::NCD * S::NHAB;
-    static constexpr int IB = XI >= 2304 ? 1 : (2304 / XI > 9 ? 9 : 2304 / XI);  // bra prim pairs per X batch
+    static constexpr int IB0 = XI >= 2304 ? 1 : (2304 / XI > 9 ? 9 : 2304 / XI);
+    static constexpr int IB = SMALL ? clampi(1024 / XI, 1, IB0) : IB0;  // bra prim pairs per X batch
     // ket transform: lane groups of NHAB lanes, each group on its own ket primitive pairs of the batch
     static constexpr int G = 64 / S::NHAB > KB ? KB : (64 / S::NHAB < 1 ? 1 : 64 / S::NHAB);
 };
 
 // Per-wavefront LDS of the R batch and the ket-transformed block, in the quartet's arithmetic
 // type.  The R recursion table and the contracted block I (always FP64) live outside it.
-template <int LA, int LB, int LC, int LD, typename RT, bool FACT = false>
+template <int LA, int LB, int LC, int LD, typename RT, bool FACT = false, bool SMALL = false>
 struct H2Smem {
     using S = HeavyShape<LA, LB, LC, LD>;
-    using C = H2Cfg<LA, LB, LC, LD>;
+    using C = H2Cfg<LA, LB, LC, LD, SMALL>;
     RT R[2][C::KB][S::NR];
     RT top[C::KB][S::L + 1];  // (-2 alpha)^j F_j per batch slot
     RT geo[C::KB][4];         // P-Q (x,y,z), lambda per batch slot
@@ -72,13 +77,13 @@ __device__ __forceinline__ void heavy2_init(int* rtab, int lane)
 
 // One contracted quartet into sI (FP64, NAB x NCD).  RT = float evaluates the primitive integrals
 // in FP32 (the Boys function at T >= 40 and the (-2 alpha)^j F_j and lambda products in FP64).
-template <int LA, int LB, int LC, int LD, typename RT, bool FACT>
-__device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD, RT, FACT>& sm,
+template <int LA, int LB, int LC, int LD, typename RT, bool FACT, bool SMALL>
+__device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD, RT, FACT, SMALL>& sm,
                                                const int* __restrict__ rtab, double* sI, int lane, int t0, int u0,
                                                int v0, int4 pi, double4 pg, int4 qi, double4 qg)
 {
     using S = HeavyShape<LA, LB, LC, LD>;
-    using C = H2Cfg<LA, LB, LC, LD>;
+    using C = H2Cfg<LA, LB, LC, LD, SMALL>;
     constexpr int NHCD = nherm(S::LCD);
     for (int e = lane; e < S::NI; e += 64) sI[e] = 0.0;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;

@@ -1,4 +1,4 @@
-// qcjk_k_heavy2_d.hip -- explicit instantiations of launch_heavy2 and launch_heavy2f (register-transform wavefront kernel, both bra transforms) for bra class dd, ket dp and dd.
+// qcjk_k_heavy2_d.hip -- explicit instantiations of the register-transform wavefront kernel launchers (both bra transforms, both LDS batch sizes) for bra class dd, ket dp and dd.
 // One translation unit per group so the kernel templates compile in parallel.
 #include "qcjk_kernels.h"
 
@@ -6,7 +6,9 @@ namespace qcjk {
 
 #define QCJK_INST(a, b, c, d)                                                                         \
     template hipError_t launch_heavy2<a, b, c, d>(const BuildArgs&, const ClassPairArgs&, hipStream_t); \
-    template hipError_t launch_heavy2f<a, b, c, d>(const BuildArgs&, const ClassPairArgs&, hipStream_t);
+    template hipError_t launch_heavy2f<a, b, c, d>(const BuildArgs&, const ClassPairArgs&, hipStream_t); \
+    template hipError_t launch_heavy2s<a, b, c, d>(const BuildArgs&, const ClassPairArgs&, hipStream_t); \
+    template hipError_t launch_heavy2fs<a, b, c, d>(const BuildArgs&, const ClassPairArgs&, hipStream_t);
 QCJK_INST(2, 2, 2, 1) QCJK_INST(2, 2, 2, 2)
 
 }  // namespace qcjk

@@ -20,6 +20,7 @@
 //   2 k_jk_heavy2  wavefront per quartet, register-resident transforms, batched R (mnd_heavy2.h).
 //   3 k_jk_lightc  lane per quartet, chunked over the ket's first component.
 //   4 k_jk_heavy2  with the sum-factorised bra transform (FACT = true).
+//   5, 6           variants 2 and 4 with small LDS batches (SMALL = true): more wavefronts per SIMD.
 // With QCJK_MIXED, variants 0, 2 and 3 evaluate small quartets' primitive integrals in FP32
 // (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.  The FP32 work
 // runs in its own instantiation of the kernel (FP32 = true), launched after the FP64 one over the
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, bool FP32, bool FACT>
+template <int LA, int LB, int LC, int LD, bool FP32, bool FACT, bool SMALL>
 __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 {
     using Dg = HeavyDigest<LA, LB, LC, LD>;
@@ -443,11 +444,11 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
     using RT = std::conditional_t<FP32, float, double>;
     __shared__ int rtab[S::NR];
     __shared__ double sI[S::NI];
-    __shared__ H2Smem<LA, LB, LC, LD, RT, FACT> sm;
+    __shared__ H2Smem<LA, LB, LC, LD, RT, FACT, SMALL> sm;
     const int lane = threadIdx.x;
     heavy2_init<LA, LB, LC, LD>(rtab, lane);
     int t0 = 0, u0 = 0, v0 = 0;
-    if (lane < H2Cfg<LA, LB, LC, LD>::G * S::NHAB) hdec(lane % S::NHAB, t0, u0, v0);
+    if (lane < H2Cfg<LA, LB, LC, LD, SMALL>::G * S::NHAB) hdec(lane % S::NHAB, t0, u0, v0);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
     UnitRange ur(C);
     for (long long u = ur.u; u < ur.end; ++u) {
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
         if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
         const bool small = (A.flags & kFlagMixed) && pg.w * qg.w * (*A.dmax) < A.mixed_tol;  // one quartet per wave
         if (small != FP32) continue;  // the other precision's quartet
-        heavy2_compute<LA, LB, LC, LD, RT, FACT>(A, sm, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        heavy2_compute<LA, LB, LC, LD, RT, FACT, SMALL>(A, sm, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
         const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
         for (int t = lane; t < Dg::NTASK; t += 64) {
             int row, col;
@@ -523,28 +524,42 @@ hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
     return hipGetLastError();
 }
 
-template <int LA, int LB, int LC, int LD>
-hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+// Wavefront-per-quartet launches: FACT selects the bra transform, SMALL the LDS batch sizes.
+template <int LA, int LB, int LC, int LD, bool FACT, bool SMALL>
+hipError_t launch_heavy2_cfg(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
     const long long cap = 256LL * 64;
     const long long blocks = c.nunits < cap ? c.nunits : cap;
-    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, false>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, FACT, SMALL>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
     if (a.flags & kFlagMixed)
-        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, false>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, FACT, SMALL>), dim3((unsigned)blocks), dim3(64), 0, s, a,
+                           c);
     return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, false, false>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2f(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    if (c.nunits <= 0) return hipSuccess;
-    const long long cap = 256LL * 64;
-    const long long blocks = c.nunits < cap ? c.nunits : cap;
-    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, true>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
-    if (a.flags & kFlagMixed)
-        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, true>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
-    return hipGetLastError();
+    return launch_heavy2_cfg<LA, LB, LC, LD, true, false>(a, c, s);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2s(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, false, true>(a, c, s);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2fs(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, true, true>(a, c, s);
 }
 
 }  // namespace qcjk

@@ -57,5 +57,9 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2f(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2s(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2fs(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
 
 }  // namespace qcjk

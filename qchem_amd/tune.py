@@ -1,9 +1,11 @@
 """Autotuner front end: per angular-momentum class, pick the faster J/K kernel variant on the
 workload at hand and cache the choice by (device architecture, class) in a JSON file.
 
-Variant 0 is the lane-per-quartet kernel (register resident) and variant 1 the wavefront-per-quartet
-kernel (LDS staged).  The engine times both on the device (qcjk_autotune); this module only persists
-and re-applies the result.
+Variants: 0 lane per quartet (register resident), 1 wavefront per quartet with LDS-staged transforms,
+2 wavefront per quartet with register transforms, 3 lane per quartet chunked over the ket's first
+component, 4 variant 2 with the factorised bra transform, 5 and 6 variants 2 and 4 with small LDS
+batches.  The engine times every variant compiled for a class on the device (qcjk_autotune); this
+module only persists and re-applies the result.
 """
 import json
 import os
