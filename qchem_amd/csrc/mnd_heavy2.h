@@ -18,6 +18,8 @@
 //                  registers, or (FACT) one axis at a time (Y[u][v] = sum_t Ex X[t][u][v],
 //                  Z[v] = sum_u Ey Y[u][v], I = sum_v Ez Z[v]) with the tables of the batch's bra
 //                  primitive pairs in LDS: fewer registers and FLOPs, more LDS.
+// FACT is a mask: bit 0 factorises the bra transform as above, bit 1 the ket transform the same way
+// (Y[nu][phi] = sum_tau (-1)^tau Ex R, ...) with the batch's ket Hermite tables in LDS.
 #pragma once
 
 #include "mnd_heavy.h"
@@ -41,7 +43,7 @@ struct H2Cfg {
 
 // Per-wavefront LDS of the R batch and the ket-transformed block, in the quartet's arithmetic
 // type.  The R recursion table and the contracted block I (always FP64) live outside it.
-template <int LA, int LB, int LC, int LD, typename RT, bool FACT = false, bool SMALL = false>
+template <int LA, int LB, int LC, int LD, typename RT, int FACT = 0, bool SMALL = false>
 struct H2Smem {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD, SMALL>;
@@ -49,7 +51,8 @@ struct H2Smem {
     RT top[C::KB][S::L + 1];  // (-2 alpha)^j F_j per batch slot
     RT geo[C::KB][4];         // P-Q (x,y,z), lambda per batch slot
     RT X[C::IB][S::NCD][S::NHAB];
-    HermiteAxis<LA, LB, RT> E[FACT ? C::IB : 1][3];  // FACT: bra Hermite tables of a batch's bra primitive pairs
+    HermiteAxis<LA, LB, RT> E[(FACT & 1) ? C::IB : 1][3];   // bra Hermite tables of a batch's bra primitive pairs
+    HermiteAxis<LC, LD, RT> Ek[(FACT & 2) ? C::KB : 1][3];  // ket Hermite tables of a batch's ket primitive pairs
 };
 
 // Reference-order index of the Cartesian component (x, y, z) of a shell of angular momentum l.
@@ -77,7 +80,7 @@ __device__ __forceinline__ void heavy2_init(int* rtab, int lane)
 
 // One contracted quartet into sI (FP64, NAB x NCD).  RT = float evaluates the primitive integrals
 // in FP32 (the Boys function at T >= 40 and the (-2 alpha)^j F_j and lambda products in FP64).
-template <int LA, int LB, int LC, int LD, typename RT, bool FACT, bool SMALL>
+template <int LA, int LB, int LC, int LD, typename RT, int FACT, bool SMALL>
 __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB, LC, LD, RT, FACT, SMALL>& sm,
                                                const int* __restrict__ rtab, double* sI, int lane, int t0, int u0,
                                                int v0, int4 pi, double4 pg, int4 qi, double4 qg)
@@ -121,6 +124,17 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     sm.geo[lane][2] = RT(Z);
                     sm.geo[lane][3] = RT(kTwoPi52 / (p * q * sqrt(pq)) * b1.w * k1.w);
                 }
+                if constexpr ((FACT & 2) != 0) {
+                    for (int tk = lane; tk < nb * 3; tk += 64) {  // ket Hermite tables, one axis per lane
+                        const int s = tk / 3, ax = tk - s * 3;
+                        const double4 k0 = A.pp0[qi.z + kl0 + s], k1 = A.pp1[qi.z + kl0 + s];
+                        const double qc = ax == 0 ? k1.x : (ax == 1 ? k1.y : k1.z);
+                        const double cd = ax == 0 ? qg.x : (ax == 1 ? qg.y : qg.z);
+                        HermiteAxis<LC, LD, RT> e;
+                        e.build(RT(0.5 / k0.w), RT(qc), RT(qc + cd));
+                        e.store(sm.Ek[s][ax]);
+                    }
+                }
                 __syncthreads();
                 static_for<0, S::L + 1>([&](auto jj) {
                     constexpr int j = S::L - decltype(jj)::value;
@@ -141,12 +155,6 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                 });
                 if (xl) {
                     for (int s = grp; s < nb; s += C::G) {
-                        const double4 k0 = A.pp0[qi.z + kl0 + s], k1 = A.pp1[qi.z + kl0 + s];
-                        HermiteAxis<LC, LD, RT> ecd[3];
-                        const RT c12 = RT(0.5 / k0.w);
-                        ecd[0].build(c12, RT(k1.x), RT(k1.x + qg.x));
-                        ecd[1].build(c12, RT(k1.y), RT(k1.y + qg.y));
-                        ecd[2].build(c12, RT(k1.z), RT(k1.z + qg.z));
                         const RT* R0 = sm.R[0][s];
                         RT rw[NHCD];
                         static_for<0, S::LCD + 1>([&](auto ta) {
@@ -160,29 +168,87 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                             });
                         });
                         const RT lam = sm.geo[s][3];
-                        static_for<0, S::NC>([&](auto cc) {
-                            constexpr int c = decltype(cc)::value;
-                            static_for<0, S::ND>([&](auto dd) {
-                                constexpr int d = decltype(dd)::value;
-                                constexpr int cx = comp(LC, c, 0), cy = comp(LC, c, 1), cz = comp(LC, c, 2);
-                                constexpr int dx = comp(LD, d, 0), dy = comp(LD, d, 1), dz = comp(LD, d, 2);
-                                RT acc = RT(0);
-                                static_for<0, cx + dx + 1>([&](auto t2) {
-                                    constexpr int tau = decltype(t2)::value;
-                                    static_for<0, cy + dy + 1>([&](auto u2) {
-                                        constexpr int nu = decltype(u2)::value;
-                                        const RT exy = ecd[0].e[cx][dx][tau] * ecd[1].e[cy][dy][nu];
-                                        static_for<0, cz + dz + 1>([&](auto v2) {
-                                            constexpr int phi = decltype(v2)::value;
-                                            const RT term = exy * ecd[2].e[cz][dz][phi] * rw[hidx(tau, nu, phi)];
-                                            if constexpr (((tau + nu + phi) & 1) != 0) acc -= term;
-                                            else acc += term;
+                        if constexpr ((FACT & 2) == 0) {
+                            const double4 k0 = A.pp0[qi.z + kl0 + s], k1 = A.pp1[qi.z + kl0 + s];
+                            HermiteAxis<LC, LD, RT> ecd[3];
+                            const RT c12 = RT(0.5 / k0.w);
+                            ecd[0].build(c12, RT(k1.x), RT(k1.x + qg.x));
+                            ecd[1].build(c12, RT(k1.y), RT(k1.y + qg.y));
+                            ecd[2].build(c12, RT(k1.z), RT(k1.z + qg.z));
+                            static_for<0, S::NC>([&](auto cc) {
+                                constexpr int c = decltype(cc)::value;
+                                static_for<0, S::ND>([&](auto dd) {
+                                    constexpr int d = decltype(dd)::value;
+                                    constexpr int cx = comp(LC, c, 0), cy = comp(LC, c, 1), cz = comp(LC, c, 2);
+                                    constexpr int dx = comp(LD, d, 0), dy = comp(LD, d, 1), dz = comp(LD, d, 2);
+                                    RT acc = RT(0);
+                                    static_for<0, cx + dx + 1>([&](auto t2) {
+                                        constexpr int tau = decltype(t2)::value;
+                                        static_for<0, cy + dy + 1>([&](auto u2) {
+                                            constexpr int nu = decltype(u2)::value;
+                                            const RT exy = ecd[0].e[cx][dx][tau] * ecd[1].e[cy][dy][nu];
+                                            static_for<0, cz + dz + 1>([&](auto v2) {
+                                                constexpr int phi = decltype(v2)::value;
+                                                const RT term = exy * ecd[2].e[cz][dz][phi] * rw[hidx(tau, nu, phi)];
+                                                if constexpr (((tau + nu + phi) & 1) != 0) acc -= term;
+                                                else acc += term;
+                                            });
+                                        });
+                                    });
+                                    xacc[c * S::ND + d] += lam * acc;
+                                });
+                            });
+                        } else {
+                            // axis by axis over the batch's ket Hermite tables in LDS:
+                            // Y[nu][phi] = sum_tau (-1)^tau Ex R, then (-1)^nu Ey, then (-1)^phi Ez
+                            const HermiteAxis<LC, LD, RT>& Kx = sm.Ek[s][0];
+                            const HermiteAxis<LC, LD, RT>& Ky = sm.Ek[s][1];
+                            const HermiteAxis<LC, LD, RT>& Kz = sm.Ek[s][2];
+                            static_for<0, LC + 1>([&](auto cxx) {
+                                constexpr int cx = decltype(cxx)::value;
+                                static_for<0, LD + 1>([&](auto dxx) {
+                                    constexpr int dx = decltype(dxx)::value;
+                                    constexpr int M = (LC - cx) + (LD - dx);
+                                    RT Y[nherm2(M)];
+                                    static_for<0, M + 1>([&](auto nn) {
+                                        constexpr int nu = decltype(nn)::value;
+                                        static_for<0, M + 1 - nu>([&](auto pp) {
+                                            constexpr int phi = decltype(pp)::value;
+                                            RT y = RT(0);
+                                            static_for<0, cx + dx + 1>([&](auto tt) {
+                                                constexpr int tau = decltype(tt)::value;
+                                                const RT term = Kx.e[cx][dx][tau] * rw[hidx(tau, nu, phi)];
+                                                if constexpr ((tau & 1) != 0) y -= term;
+                                                else y += term;
+                                            });
+                                            Y[hidx2(nu, phi)] = y;
+                                        });
+                                    });
+                                    static_for<0, LC - cx + 1>([&](auto cyy) {
+                                        constexpr int cy = decltype(cyy)::value, cz = LC - cx - cy;
+                                        static_for<0, LD - dx + 1>([&](auto dyy) {
+                                            constexpr int dy = decltype(dyy)::value, dz = LD - dx - dy;
+                                            RT acc = RT(0);
+                                            static_for<0, cz + dz + 1>([&](auto pp) {
+                                                constexpr int phi = decltype(pp)::value;
+                                                RT z = RT(0);
+                                                static_for<0, cy + dy + 1>([&](auto nn) {
+                                                    constexpr int nu = decltype(nn)::value;
+                                                    const RT term = Ky.e[cy][dy][nu] * Y[hidx2(nu, phi)];
+                                                    if constexpr ((nu & 1) != 0) z -= term;
+                                                    else z += term;
+                                                });
+                                                const RT term = Kz.e[cz][dz][phi] * z;
+                                                if constexpr ((phi & 1) != 0) acc -= term;
+                                                else acc += term;
+                                            });
+                                            constexpr int c = cidx(LC, cy, cz), d = cidx(LD, dy, dz);
+                                            xacc[c * S::ND + d] += lam * acc;
                                         });
                                     });
                                 });
-                                xacc[c * S::ND + d] += lam * acc;
                             });
-                        });
+                        }
                     }
                 }
             }
@@ -193,7 +259,7 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
                     static_for<0, S::NCD>([&](auto k) { sm.X[ii][decltype(k)::value][hl] = xacc[decltype(k)::value]; });
             }
         }
-        if constexpr (!FACT) {
+        if constexpr ((FACT & 1) == 0) {
             __syncthreads();
             for (int pr = lane; pr < nib * S::NCD; pr += 64) {
                 const int ii = pr / S::NCD, cd = pr - ii * S::NCD;

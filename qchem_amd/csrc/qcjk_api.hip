@@ -33,9 +33,10 @@ constexpr double kFp32MaxLog10 = 30.0;  // FP32 quartets only where (2 alpha)^L 
 // Variants: 0 = lane per quartet (64 ket pairs per unit), 1 = wavefront per quartet with LDS
 // transforms, 2 = wavefront per quartet with register transforms (1 ket pair per unit),
 // 3 = lane per quartet chunked over the ket's first component (64 per unit), 4 = variant 2 with the
-// sum-factorised bra transform (1 per unit), 5 and 6 = variants 2 and 4 with small LDS batches.
-constexpr int kNumVariants = 7;
-constexpr int kTile[kNumVariants] = {64, 1, 1, 64, 1, 1, 1};
+// sum-factorised bra transform (1 per unit), 5 and 6 = variants 2 and 4 with small LDS batches,
+// 7 and 8 = variant 2 with both transforms sum-factorised, big and small LDS batches.
+constexpr int kNumVariants = 9;
+constexpr int kTile[kNumVariants] = {64, 1, 1, 64, 1, 1, 1, 1, 1};
 
 struct ClassPairWork {
     int bra, ket;
@@ -61,7 +62,9 @@ struct ClassPairWork {
         case 3: return info->lightc;
         case 4: return info->heavy2f;
         case 5: return info->heavy2s;
-        default: return info->heavy2fs;
+        case 6: return info->heavy2fs;
+        case 7: return info->heavy2k;
+        default: return info->heavy2ks;
         }
     }
     int unit_list(int v) const { return (v == 0 || v == 3) ? 0 : 1; }
@@ -912,7 +915,8 @@ int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* o
     std::vector<qcjk_class_stat> res;
     for (size_t i = 0; i < h->work.size() && rc == QCJK_OK; ++i) {
         ClassPairWork& w = h->work[i];
-        double best[kNumVariants] = {1e300, 1e300, 1e300, 1e300, 1e300, 1e300, 1e300};
+        double best[kNumVariants];
+        for (double& b : best) b = 1e300;
         for (int v = 0; v < kNumVariants && rc == QCJK_OK; ++v) {
             if (!w.has(v)) continue;
             const ClassPairArgs c = class_args(h, w, v);

@@ -16,7 +16,8 @@ constexpr ClassKernelInfo make_info()
     return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1,
             default_light(LA, LB, LC, LD) ? light : &launch_heavy2<LA, LB, LC, LD>, light,
             &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>, lightc, &launch_heavy2f<LA, LB, LC, LD>,
-            &launch_heavy2s<LA, LB, LC, LD>, &launch_heavy2fs<LA, LB, LC, LD>};
+            &launch_heavy2s<LA, LB, LC, LD>, &launch_heavy2fs<LA, LB, LC, LD>, &launch_heavy2k<LA, LB, LC, LD>,
+            &launch_heavy2ks<LA, LB, LC, LD>};
 }
 #define QCJK_CLASS(LA, LB, LC, LD) make_info<LA, LB, LC, LD>()
 

@@ -66,6 +66,7 @@ struct ClassKernelInfo {
     ClassLauncher lightc;  // lane per quartet, chunked over the ket's first component (may be null)
     ClassLauncher heavy2f; // heavy2 with the sum-factorised bra transform
     ClassLauncher heavy2s, heavy2fs;  // heavy2 / heavy2f with small LDS batches
+    ClassLauncher heavy2k, heavy2ks;  // heavy2 with both transforms factorised, big / small batches
 };
 
 // Table of the 21 bra-class >= ket-class combinations (qcjk_eri.hip).

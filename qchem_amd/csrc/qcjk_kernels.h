@@ -21,6 +21,7 @@
 //   3 k_jk_lightc  lane per quartet, chunked over the ket's first component.
 //   4 k_jk_heavy2  with the sum-factorised bra transform (FACT = true).
 //   5, 6           variants 2 and 4 with small LDS batches (SMALL = true): more wavefronts per SIMD.
+//   7, 8           k_jk_heavy2 with both transforms sum-factorised (FACT = 3), big / small batches.
 // With QCJK_MIXED, variants 0, 2 and 3 evaluate small quartets' primitive integrals in FP32
 // (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.  The FP32 work
 // runs in its own instantiation of the kernel (FP32 = true), launched after the FP64 one over the
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, bool FP32, bool FACT, bool SMALL>
+template <int LA, int LB, int LC, int LD, bool FP32, int FACT, bool SMALL>
 __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 {
     using Dg = HeavyDigest<LA, LB, LC, LD>;
@@ -524,8 +525,9 @@ hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
     return hipGetLastError();
 }
 
-// Wavefront-per-quartet launches: FACT selects the bra transform, SMALL the LDS batch sizes.
-template <int LA, int LB, int LC, int LD, bool FACT, bool SMALL>
+// Wavefront-per-quartet launches: FACT selects the factorised transforms (bit 0 bra, bit 1 ket),
+// SMALL the LDS batch sizes.
+template <int LA, int LB, int LC, int LD, int FACT, bool SMALL>
 hipError_t launch_heavy2_cfg(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
@@ -541,25 +543,37 @@ hipError_t launch_heavy2_cfg(const BuildArgs& a, const ClassPairArgs& c, hipStre
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, false, false>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, false>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2f(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, true, false>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 1, false>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2s(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, false, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, true>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2fs(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, true, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 1, true>(a, c, s);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2k(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, 3, false>(a, c, s);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2ks(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, 3, true>(a, c, s);
 }
 
 }  // namespace qcjk

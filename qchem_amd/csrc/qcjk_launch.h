@@ -61,5 +61,9 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2s(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2fs(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2k(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2ks(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
 
 }  // namespace qcjk

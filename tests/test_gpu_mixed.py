@@ -57,7 +57,7 @@ def test_mixed_zero_tol_is_fp64(require_gpu):
     assert abs_err(J1, J0) < 1e-13 and abs_err(K1, K0) < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 2, 3, 4, 5, 6, 7, 8])
 def test_fp32_kernels_every_class(require_gpu, variant):
     """mixed_tol = 1e30 sends every quartet of the chosen variant to its FP32 path."""
     m = small_cluster(5, 51)
