@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the untuned default kernel variant per class")
     ap.add_argument("--retune", action="store_true", help="re-run the per-class kernel-variant autotuner")
     ap.add_argument("--batch-scf", action="store_true", help="also run the 512-molecule batched RHF (minutes)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="concurrent class streams per build (1-4; 0 = the engine's default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a copy kernel of known size (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -249,21 +251,34 @@ def main():
     elif not args.no_tune:  # the committed per-class choices for this workload on gfx950
         table = tune.apply(eng, workload=workload_key)
         tuning = {"source": "qchem_amd/tuning/gfx950.json", "classes": table}
+    if args.streams:
+        eng.set_streams(args.streams)
     st = eng.build_device(D, J, K, flags, stream=stream, stats=True)  # counts + first build
     for _ in range(args.warmup):
         eng.build_device(D, J, K, flags, stream=stream)
-    torch.cuda.synchronize()
-    eng.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eng.build_device(D, J, K, flags, stream=stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    rep = eng.class_report()
-    eng.profile(False)
     elapsed = t1 - t0
     ms = 1e3 * elapsed / args.steps
+    # Per-class kernel durations: the same K builds again with the classes one after another on the
+    # build stream (under the default overlap a class's events would include the time it shares
+    # the device with its neighbours), HIP events around every class launch.
+    nstreams = eng.streams()
+    eng.set_streams(1)
+    eng.profile(True)
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.build_device(D, J, K, flags, stream=stream)
+    torch.cuda.synchronize()
+    ms_serial = 1e3 * (time.perf_counter() - s0) / args.steps
+    rep = eng.class_report()
+    eng.profile(False)
+    eng.set_streams(nstreams)
     quartets = st.quartets_computed
     value = quartets * args.steps / elapsed
 
@@ -282,7 +297,9 @@ def main():
                                            8: ", factorised bra and ket, small LDS>"}.get(
                                                dom["wavefront_per_quartet"], ">"),
             "kernel_ms_per_launch": per_launch_s * 1e3, "kernel_flops_per_launch": per_launch_flops,
-            "build_tflops": total_flops / (ms / 1e3) / 1e12}
+            "build_tflops": total_flops / (ms / 1e3) / 1e12,
+            "measured": "HIP events around each class launch, %d builds with the classes in sequence on one "
+                        "stream (%.2f ms per build)" % (args.steps, ms_serial)}
 
     cpu = None if args.no_cpu else cpu_baseline(mol, Dh)
     scf = mixed = batched = None
@@ -303,7 +320,7 @@ def main():
            "config": {"workload": "FP64 J+K Fock build, %s, n=%d basis functions, %d shell quartets (8-fold)"
                                   % (mol["name"], n, quartets),
                       "n_basis": n, "quartets_per_build": quartets, "fock_build_s": ms / 1e3,
-                      "parallelism": "single GPU"},
+                      "class_streams": nstreams, "parallelism": "single GPU"},
            "roofline": roof, "cpu_baseline": cpu, "scf": scf, "mixed_precision": mixed, "batched": batched,
            "tuning": tuning,
            "classes": [{k: r[k] for k in ("la", "lb", "lc", "ld", "wavefront_per_quartet", "quartets")} |

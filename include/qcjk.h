@@ -171,6 +171,14 @@ int qcjk_class_report(qcjk_handle* h, qcjk_class_stat* out, int max, int* count)
  * device_ms and the best other variant's time in alt_ms.  Results are cached by the caller
  * (qchem_amd.tune) and re-applied with qcjk_set_variant. */
 int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant);
+
+/* Number of streams a build spreads its class launches over (no reference counterpart): 1 = all
+ * on the build stream in class order; 2-4 = round-robin over the build stream and side streams
+ * forked from it and joined back before the symmetrisation, so classes overlap (default 3).
+ * Per-class profiling events are recorded on the class's own stream: under overlap a class's time
+ * includes the time it shares the device with its neighbours. */
+int qcjk_set_streams(qcjk_handle* h, int nstreams);
+int qcjk_streams(const qcjk_handle* h);  /* the current count, -1 for a null handle */
 int qcjk_autotune(qcjk_handle* h, const double* dD, int reps, qcjk_class_stat* out, int max, int* count);
 
 void qcjk_destroy(qcjk_handle* h);

@@ -91,7 +91,8 @@ class _ScfResult(ctypes.Structure):
 
 QCJK_SYMBOLS = ["qcjk_create", "qcjk_create_batch", "qcjk_nbf", "qcjk_layout", "qcjk_build", "qcjk_build_device",
                 "qcjk_graph_build", "qcjk_buffers", "qcjk_one_electron", "qcjk_one_electron_mol", "qcjk_schwarz",
-                "qcjk_profile", "qcjk_class_report", "qcjk_set_variant", "qcjk_autotune", "qcjk_destroy",
+                "qcjk_profile", "qcjk_class_report", "qcjk_set_variant", "qcjk_set_streams", "qcjk_streams", "qcjk_autotune",
+                "qcjk_destroy",
                 "qcjk_strerror"]
 HOST_SYMBOLS = ["qchost_basis_prepare", "qcscf_rhf", "qcscf_rhf_batch", "qcscf_sad", "qchost_last_error"]
 
@@ -143,6 +144,8 @@ def lib():
         L.qcjk_profile.argtypes = [vp, ctypes.c_int]
         L.qcjk_class_report.argtypes = [vp, ctypes.POINTER(ClassStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.qcjk_set_variant.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.qcjk_set_streams.argtypes = [vp, ctypes.c_int]
+        L.qcjk_streams.argtypes = [vp]
         L.qcjk_autotune.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ClassStat), ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int)]
         L.qcjk_destroy.argtypes = [vp]
@@ -343,6 +346,16 @@ class JKEngine:
         rc = lib().qcjk_set_variant(self.h, la, lb, lc, ld, variant)
         if rc != 0:
             raise EngineError("qcjk_set_variant", rc)
+
+    def set_streams(self, nstreams):
+        """Spread each build's class launches over nstreams concurrent streams (1-4)."""
+        rc = lib().qcjk_set_streams(self.h, int(nstreams))
+        if rc != 0:
+            raise EngineError("qcjk_set_streams", rc)
+
+    def streams(self):
+        """Concurrent class streams per build."""
+        return int(lib().qcjk_streams(self.h))
 
     def autotune(self, dD, reps=3):
         """Time every kernel variant per class on device density dD; keep the fastest (see tune.py)."""

@@ -153,10 +153,15 @@ __global__ void k_finalize(const double* __restrict__ Jt, const double* __restri
 
 }  // namespace
 
+constexpr int kMaxStreams = 4;  // class launches spread over the build stream and up to 3 side streams
+
 struct qcjk_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int nstreams = 1;                            // concurrent class streams (qcjk_set_streams)
+    hipStream_t side[kMaxStreams - 1] = {};      // default priority, non-blocking
+    hipEvent_t fork = nullptr, join[kMaxStreams - 1] = {};
     qcjk_opts opts{};
     int n = 0, nshell = 0;           // basis functions and shells over all molecules
     int nmol = 1;
@@ -218,6 +223,11 @@ static void free_handle(qcjk_handle* h)
     for (hipEvent_t e : h->pev) hipEventDestroy(e);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->fork) hipEventDestroy(h->fork);
+    for (int k = 0; k < kMaxStreams - 1; ++k) {
+        if (h->join[k]) hipEventDestroy(h->join[k]);
+        if (h->side[k]) hipStreamDestroy(h->side[k]);
+    }
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
@@ -253,6 +263,12 @@ static int create_impl(const qcjk_basis* mols, int nmol, const qcjk_opts* o, qcj
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&h->ev0));
     HIPCHK(hipEventCreate(&h->ev1));
+    HIPCHK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+    for (int k = 0; k < kMaxStreams - 1; ++k) {
+        HIPCHK(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&h->join[k], hipEventDisableTiming));
+    }
+    h->nstreams = kMaxStreams - 1;
     const bool force_heavy = std::getenv("QCJK_FORCE_HEAVY") != nullptr;
 
     // concatenate the molecules' shells; basis functions are numbered within each molecule
@@ -568,7 +584,7 @@ static int fold_profile(qcjk_handle* h)
     h->pending = false;
     const int nctr = (int)(3 * h->work.size() + 3);
     std::vector<unsigned long long> ctr(nctr);
-    HIPCHK(hipEventSynchronize(h->pev.back()));
+    for (size_t i = 0; i < h->work.size(); ++i) HIPCHK(hipEventSynchronize(h->pev[2 * i + 1]));  // every stream
     HIPCHK(hipMemcpy(ctr.data(), h->d_counters, nctr * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < h->work.size(); ++i) {
         float ms = 0.f;
@@ -678,15 +694,29 @@ static int enqueue_build(qcjk_handle* h, const double* dD, double* dJ, double* d
         if (rc != QCJK_OK) return rc;
     }
     if (flags & (QCJK_J | QCJK_K)) {
+        // Class launches round-robin over ns streams forked from and joined back into s, so one
+        // class's tail overlaps the next class's start (every kernel adds into Jt/Kt atomically).
+        // Per-class profiling events go on the class's own stream.
+        const int ns = h->nstreams;
+        if (ns > 1) {
+            HIPCHK(hipEventRecord(h->fork, s));
+            for (int k = 0; k < ns - 1; ++k) HIPCHK(hipStreamWaitEvent(h->side[k], h->fork, 0));
+        }
         BuildArgs a = make_args(h, flags);
         for (size_t i = 0; i < h->work.size(); ++i) {
             const ClassPairWork& w = h->work[i];
             const ClassPairArgs c = class_args(h, w, w.variant);
+            const int k = (int)(i % (size_t)ns);
+            hipStream_t si = k == 0 ? s : h->side[k - 1];
             a.counters = h->d_counters + 3 * i;
             a.flags = w.fp32_ok ? flags : (flags & ~(unsigned)QCJK_MIXED);
-            if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i], s));
-            HIPCHK(w.launcher(w.variant)(a, c, s));
-            if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i + 1], s));
+            if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i], si));
+            HIPCHK(w.launcher(w.variant)(a, c, si));
+            if (h->profiling) HIPCHK(hipEventRecord(h->pev[2 * i + 1], si));
+        }
+        for (int k = 0; k < ns - 1; ++k) {
+            HIPCHK(hipEventRecord(h->join[k], h->side[k]));
+            HIPCHK(hipStreamWaitEvent(s, h->join[k], 0));
         }
     }
     hipLaunchKernelGGL(k_finalize, elem_grid(h), dim3(256), 0, s, h->d_Jt, h->d_Kt, h->d_ns, h->d_mol,
@@ -888,6 +918,16 @@ int qcjk_schwarz(qcjk_handle* h, double* Q)
 }
 
 void qcjk_destroy(qcjk_handle* h) { free_handle(h); }
+
+int qcjk_set_streams(qcjk_handle* h, int nstreams)
+{
+    if (!h || nstreams < 1 || nstreams > kMaxStreams) return QCJK_ERR_ARG;
+    if (h->nstreams != nstreams) h->gD = nullptr;  // recapture the graph on its next use
+    h->nstreams = nstreams;
+    return QCJK_OK;
+}
+
+int qcjk_streams(const qcjk_handle* h) { return h ? h->nstreams : -1; }
 
 int qcjk_set_variant(qcjk_handle* h, int la, int lb, int lc, int ld, int variant)
 {

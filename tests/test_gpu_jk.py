@@ -150,6 +150,42 @@ def test_every_variant_of_every_class(require_gpu):
         assert rel_err(J, Jr) < TOL and rel_err(K, Kr) < TOL, variant
 
 
+def test_class_streams_agree(require_gpu):
+    """Class launches spread over 1-4 concurrent streams give the same J/K (oracle bar, and 1e-12
+    between stream counts), also with per-class profiling on and through a captured graph."""
+    import torch
+    m = small_cluster(6, 71)
+    B = qa.Basis(m["shells"])
+    D = rand_density(B.nbf, 71)
+    ob = oracle.Basis(m["shells"])
+    Jr, Kr, _ = ob.jk_direct(D)
+    eng = qa.JKEngine(B)
+    eng.set_streams(1)
+    J1, K1, _ = eng.build(D)
+    assert rel_err(J1, Jr) < TOL and rel_err(K1, Kr) < TOL
+    for ns in (2, 3, 4):
+        eng.set_streams(ns)
+        J, K, _ = eng.build(D)
+        assert rel_err(J, J1) < 1e-12 and rel_err(K, K1) < 1e-12, ns
+        eng.profile(True)
+        for _ in range(2):
+            J, K, _ = eng.build(D)
+        rep = eng.class_report()
+        eng.profile(False)
+        assert rep and all(r["launches"] == 2 and r["device_ms"] > 0 for r in rep), ns
+        assert rel_err(J, J1) < 1e-12 and rel_err(K, K1) < 1e-12, ns
+        dD = torch.from_numpy(D).cuda()
+        dJ, dK = torch.empty_like(dD), torch.empty_like(dD)
+        for _ in range(3):  # first call captures, the rest replay
+            eng.graph_build(dD, dJ, dK)
+            torch.cuda.synchronize()
+            assert rel_err(dJ.cpu().numpy(), J1) < 1e-12 and rel_err(dK.cpu().numpy(), K1) < 1e-12, ns
+    assert eng.streams() == 4
+    with pytest.raises(qa.EngineError):
+        eng.set_streams(5)
+    assert qa.JKEngine(B).streams() == 3  # the default
+
+
 def test_autotune_keeps_results(require_gpu):
     import torch
     m = small_cluster(6, 61)
