@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 METRIC = "Fock-build s & ERI quartets/s; SCF time-to-1e-6 Eh: mixed+learned vs FP64 GPU"
 VARIANT_KERNEL = {0: "light", 1: "heavy", 2: "heavy2", 3: "lightc", 4: "heavy2", 5: "heavy2", 6: "heavy2", 7: "heavy2",
-                  8: "heavy2"}
+                  8: "heavy2", 9: "heavy2", 10: "heavy2"}
 
 
 def cpu_baseline(mol, D, budget_s=15.0, threads=None):
@@ -294,7 +294,8 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(kname),
             "kernel": kname.rstrip(",") + {4: ", factorised bra>", 5: ", small LDS>",
                                            6: ", factorised bra, small LDS>", 7: ", factorised bra and ket>",
-                                           8: ", factorised bra and ket, small LDS>"}.get(
+                                           8: ", factorised bra and ket, small LDS>", 9: ", lane R>",
+                                           10: ", factorised bra, lane R>"}.get(
                                                dom["wavefront_per_quartet"], ">"),
             "kernel_ms_per_launch": per_launch_s * 1e3, "kernel_flops_per_launch": per_launch_flops,
             "build_tflops": total_flops / (ms / 1e3) / 1e12,

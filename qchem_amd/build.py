@@ -28,7 +28,8 @@ HIP_FLAGS = [
     "-I" + INCLUDE, "-I" + CSRC,
 ]
 # longest compiles first (they bound the parallel build)
-HIP_SOURCES = ["qcjk_k_lightc_b.hip", "qcjk_k_light_c.hip", "qcjk_k_light_b.hip", "qcjk_k_lightc.hip", "qcjk_k_light_a.hip",
+HIP_SOURCES = ["qcjk_k_heavy2r_c.hip", "qcjk_k_heavy2r_d.hip", "qcjk_k_heavy2r_b.hip", "qcjk_k_heavy2r_a.hip",
+               "qcjk_k_lightc_b.hip", "qcjk_k_light_c.hip", "qcjk_k_light_b.hip", "qcjk_k_lightc.hip", "qcjk_k_light_a.hip",
                "qcjk_k_heavy2_c.hip", "qcjk_k_heavy2_d.hip", "qcjk_k_heavy2_b.hip", "qcjk_k_heavy2_a.hip",
                "qcjk_k_heavy.hip", "qcjk_eri.hip", "qcjk_setup.hip", "qcjk_api.hip"]
 ML_SOURCE = "qcml.hip"

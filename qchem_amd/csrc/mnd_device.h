@@ -217,17 +217,11 @@ struct HermiteAxis {
 // R^{(0)}_{tuv}, t+u+v <= L, triangular storage (hidx), computed in place level by level.
 // R^{(j)}_{t,u,v} = (t-1) R^{(j+1)}_{t-2,u,v} + X R^{(j+1)}_{t-1,u,v}  (t>0; else the same on u, v)
 // ---------------------------------------------------------------------------------------------
+// top[j] = (-2 alpha)^j F_j(T) are the R^{(j)}_{000} that start each level.
 template <int L, typename Rt>
-QC_HD void rtuv(Rt alpha, Rt X, Rt Y, Rt Z, const Rt* F, Rt* R)
+QC_HD void rtuv_top(const Rt* top, Rt X, Rt Y, Rt Z, Rt* R)
 {
-    Rt m2a = Rt(-2) * alpha;
-    Rt pw[L + 1];
-    pw[0] = Rt(1);
-    static_for<1, L + 1>([&](auto jj) {
-        constexpr int j = decltype(jj)::value;
-        pw[j] = pw[j - 1] * m2a;
-    });
-    R[0] = pw[L] * F[L];
+    R[0] = top[L];
     static_for<0, L>([&](auto jj) {
         constexpr int j = L - 1 - decltype(jj)::value;  // level being built
         static_for<0, L - j>([&](auto ss) {
@@ -252,8 +246,21 @@ QC_HD void rtuv(Rt alpha, Rt X, Rt Y, Rt Z, const Rt* F, Rt* R)
                 });
             });
         });
-        R[0] = pw[j] * F[j];
+        R[0] = top[j];
     });
+}
+
+template <int L, typename Rt>
+QC_HD void rtuv(Rt alpha, Rt X, Rt Y, Rt Z, const Rt* F, Rt* R)
+{
+    const Rt m2a = Rt(-2) * alpha;
+    Rt pw = Rt(1), top[L + 1];
+    static_for<0, L + 1>([&](auto jj) {
+        constexpr int j = decltype(jj)::value;
+        top[j] = pw * F[j];
+        pw *= m2a;
+    });
+    rtuv_top<L, Rt>(top, X, Y, Z, R);
 }
 
 // ---------------------------------------------------------------------------------------------

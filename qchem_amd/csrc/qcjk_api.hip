@@ -34,9 +34,10 @@ constexpr double kFp32MaxLog10 = 30.0;  // FP32 quartets only where (2 alpha)^L 
 // transforms, 2 = wavefront per quartet with register transforms (1 ket pair per unit),
 // 3 = lane per quartet chunked over the ket's first component (64 per unit), 4 = variant 2 with the
 // sum-factorised bra transform (1 per unit), 5 and 6 = variants 2 and 4 with small LDS batches,
-// 7 and 8 = variant 2 with both transforms sum-factorised, big and small LDS batches.
-constexpr int kNumVariants = 9;
-constexpr int kTile[kNumVariants] = {64, 1, 1, 64, 1, 1, 1, 1, 1};
+// 7 and 8 = variant 2 with both transforms sum-factorised, big and small LDS batches, 9 and 10 =
+// variants 2 and 4 in lane-R form (whole R tables per lane).
+constexpr int kNumVariants = 11;
+constexpr int kTile[kNumVariants] = {64, 1, 1, 64, 1, 1, 1, 1, 1, 1, 1};
 
 struct ClassPairWork {
     int bra, ket;
@@ -64,7 +65,9 @@ struct ClassPairWork {
         case 5: return info->heavy2s;
         case 6: return info->heavy2fs;
         case 7: return info->heavy2k;
-        default: return info->heavy2ks;
+        case 8: return info->heavy2ks;
+        case 9: return info->heavy2r;
+        default: return info->heavy2rf;
         }
     }
     int unit_list(int v) const { return (v == 0 || v == 3) ? 0 : 1; }

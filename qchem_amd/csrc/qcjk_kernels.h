@@ -22,6 +22,7 @@
 //   4 k_jk_heavy2  with the sum-factorised bra transform (FACT = true).
 //   5, 6           variants 2 and 4 with small LDS batches (SMALL = true): more wavefronts per SIMD.
 //   7, 8           k_jk_heavy2 with both transforms sum-factorised (FACT = 3), big / small batches.
+//   9, 10          k_jk_heavy2 in lane-R form (RL = true), bra transform in registers / factorised.
 // With QCJK_MIXED, variants 0, 2 and 3 evaluate small quartets' primitive integrals in FP32
 // (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.  The FP32 work
 // runs in its own instantiation of the kernel (FP32 = true), launched after the FP64 one over the
@@ -437,19 +438,21 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, bool FP32, int FACT, bool SMALL>
+template <int LA, int LB, int LC, int LD, bool FP32, int FACT, bool SMALL, bool RL>
 __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 {
     using Dg = HeavyDigest<LA, LB, LC, LD>;
     using S = HeavyShape<LA, LB, LC, LD>;
+    using Cf = H2Cfg<LA, LB, LC, LD, SMALL>;
     using RT = std::conditional_t<FP32, float, double>;
-    __shared__ int rtab[S::NR];
+    using SM = std::conditional_t<RL, H2SmemRL<LA, LB, LC, LD, RT, FACT>, H2Smem<LA, LB, LC, LD, RT, FACT, SMALL>>;
+    __shared__ int rtab[RL ? 1 : S::NR];
     __shared__ double sI[S::NI];
-    __shared__ H2Smem<LA, LB, LC, LD, RT, FACT, SMALL> sm;
+    __shared__ SM sm;
     const int lane = threadIdx.x;
-    heavy2_init<LA, LB, LC, LD>(rtab, lane);
+    if constexpr (!RL) heavy2_init<LA, LB, LC, LD>(rtab, lane);
     int t0 = 0, u0 = 0, v0 = 0;
-    if (lane < H2Cfg<LA, LB, LC, LD, SMALL>::G * S::NHAB) hdec(lane % S::NHAB, t0, u0, v0);
+    if (lane < (RL ? Cf::GL : Cf::G) * S::NHAB) hdec(lane % S::NHAB, t0, u0, v0);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
     UnitRange ur(C);
     for (long long u = ur.u; u < ur.end; ++u) {
@@ -463,7 +466,8 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
         if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
         const bool small = (A.flags & kFlagMixed) && pg.w * qg.w * (*A.dmax) < A.mixed_tol;  // one quartet per wave
         if (small != FP32) continue;  // the other precision's quartet
-        heavy2_compute<LA, LB, LC, LD, RT, FACT, SMALL>(A, sm, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        if constexpr (RL) heavy2rl_compute<LA, LB, LC, LD, RT, FACT>(A, sm, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        else heavy2_compute<LA, LB, LC, LD, RT, FACT, SMALL>(A, sm, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
         const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
         for (int t = lane; t < Dg::NTASK; t += 64) {
             int row, col;
@@ -527,16 +531,17 @@ hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
 
 // Wavefront-per-quartet launches: FACT selects the factorised transforms (bit 0 bra, bit 1 ket),
 // SMALL the LDS batch sizes.
-template <int LA, int LB, int LC, int LD, int FACT, bool SMALL>
+template <int LA, int LB, int LC, int LD, int FACT, bool SMALL, bool RL = false>
 hipError_t launch_heavy2_cfg(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
     const long long cap = 256LL * 64;
     const long long blocks = c.nunits < cap ? c.nunits : cap;
-    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, FACT, SMALL>), dim3((unsigned)blocks), dim3(64), 0, s, a, c);
+    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, FACT, SMALL, RL>), dim3((unsigned)blocks), dim3(64), 0, s,
+                       a, c);
     if (a.flags & kFlagMixed)
-        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, FACT, SMALL>), dim3((unsigned)blocks), dim3(64), 0, s, a,
-                           c);
+        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, FACT, SMALL, RL>), dim3((unsigned)blocks), dim3(64), 0,
+                           s, a, c);
     return hipGetLastError();
 }
 
@@ -574,6 +579,18 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2ks(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     return launch_heavy2_cfg<LA, LB, LC, LD, 3, true>(a, c, s);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2r(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, false, true>(a, c, s);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_heavy2rf(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+{
+    return launch_heavy2_cfg<LA, LB, LC, LD, 1, false, true>(a, c, s);
 }
 
 }  // namespace qcjk

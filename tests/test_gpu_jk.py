@@ -139,7 +139,7 @@ def test_every_variant_of_every_class(require_gpu):
     Jr, Kr, _ = ob.jk_direct(D)
     classes = [(a, b, c, d) for a in range(3) for b in range(a + 1) for c in range(3) for d in range(c + 1)
                if (a, b) >= (c, d) or a * 3 + b >= c * 3 + d]
-    for variant in (0, 1, 2, 3, 4, 5, 6, 7, 8):
+    for variant in range(11):
         eng = qa.JKEngine(B)
         for (a, b, c, d) in classes:
             try:

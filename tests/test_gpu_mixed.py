@@ -57,7 +57,7 @@ def test_mixed_zero_tol_is_fp64(require_gpu):
     assert abs_err(J1, J0) < 1e-13 and abs_err(K1, K0) < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [0, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_fp32_kernels_every_class(require_gpu, variant):
     """mixed_tol = 1e30 sends every quartet of the chosen variant to its FP32 path."""
     m = small_cluster(5, 51)
@@ -77,7 +77,7 @@ def test_fp32_kernels_every_class(require_gpu, variant):
     eJ, eK = abs_err(J, Jr), abs_err(K, Kr)
     print("variant %d (%d classes): fp32 fraction %.3f, J err %.2e K err %.2e" % (variant, len(used), frac, eJ, eK))
     assert st.quartets_fp32 > 0
-    if variant in (2, 4, 5, 6):
+    if variant in (2, 4, 5, 6, 7, 8, 9, 10):  # every class on this variant
         assert st.quartets_fp32 == st.quartets_computed
     assert eJ < 1e-5 and eK < 1e-5
     assert eJ > 1e-12  # the FP32 path really ran
