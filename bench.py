@@ -295,7 +295,7 @@ def main():
             "kernel": kname.rstrip(",") + {4: ", factorised bra>", 5: ", small LDS>",
                                            6: ", factorised bra, small LDS>", 7: ", factorised bra and ket>",
                                            8: ", factorised bra and ket, small LDS>", 9: ", lane R>",
-                                           10: ", factorised bra, lane R>"}.get(
+                                           10: ", lane R, narrow batches>"}.get(
                                                dom["wavefront_per_quartet"], ">"),
             "kernel_ms_per_launch": per_launch_s * 1e3, "kernel_flops_per_launch": per_launch_flops,
             "build_tflops": total_flops / (ms / 1e3) / 1e12,

@@ -162,9 +162,9 @@ int qcjk_class_report(qcjk_handle* h, qcjk_class_stat* out, int max, int* count)
  * LDS-staged Hermite transforms, 2 = wavefront per quartet with register-resident transforms,
  * 4 = variant 2 with the bra transform factorised axis by axis over LDS Hermite tables, 5 and 6 =
  * variants 2 and 4 with small LDS batches (more wavefronts per SIMD), 7 and 8 = variant 2 with both
- * the bra and the ket transform factorised, big and small LDS batches, 9 and 10 = variants 2 and 4
- * with the R tables of a batch of (bra, ket) primitive pairs built one per lane (1, 2, 4-10 exist
- * for every class), 3 = lane per quartet chunked over the ket's first Cartesian component (fewer live
+ * the bra and the ket transform factorised, big and small LDS batches, 9 and 10 = variant 2 with
+ * the R tables of a batch of (bra, ket) primitive pairs built one per lane, batches of ~16 KB, and
+ * of ~8 KB with at most 256 registers (1, 2, 4-10 exist for every class), 3 = lane per quartet chunked over the ket's first Cartesian component (fewer live
  * accumulators; L <= 4 classes with a p or d ket, and (dp|ds)).  qcjk_autotune times every variant
  * of every class present in this
  * basis on the given DEVICE density (reps timed launches each, after one warm-up), keeps the

@@ -46,8 +46,10 @@ struct H2Cfg {
     static constexpr int IB = SMALL ? clampi(1024 / XI, 1, IB0) : IB0;  // bra prim pairs per X batch
     // ket transform: lane groups of NHAB lanes, each group on its own ket primitive pairs of the batch
     static constexpr int G = 64 / S::NHAB > KB ? KB : (64 / S::NHAB < 1 ? 1 : 64 / S::NHAB);
-    // lane-R form: flattened (bra, ket) primitive pairs per R batch, about 16 KB of FP64 R tables
+    // lane-R form: flattened (bra, ket) primitive pairs per R batch, about 16 KB of FP64 R tables,
+    // or (narrow) about 8 KB
     static constexpr int KBL = clampi(16384 / (8 * S::NR), 16, 64);
+    static constexpr int KBLN = clampi(8192 / (8 * (S::NR | 1)), 8, 64);
     static constexpr int GL = 64 / S::NHAB < 1 ? 1 : 64 / S::NHAB;
 };
 
@@ -67,15 +69,16 @@ struct H2Smem {
 
 // Lane-R form: R^{(0)} tables of the flattened batch, per slot its bra primitive pair (within the X
 // batch), lambda and the ket Hermite inputs 1/(2q), QC.
-template <int LA, int LB, int LC, int LD, typename RT, int FACT = 0>
+template <int LA, int LB, int LC, int LD, typename RT, int FACT = 0, bool NARROW = false>
 struct H2SmemRL {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD, false>;
-    static constexpr int NRP = S::NR | 1;  // odd row length
-    RT R[C::KBL][NRP];
-    double kq[C::KBL][4];  // 1/(2q), QC (x,y,z)
-    RT lam[C::KBL];
-    int sii[C::KBL];
+    static constexpr int KBL = NARROW ? C::KBLN : C::KBL;  // slots per R batch
+    static constexpr int NRP = S::NR | 1;                // odd row length
+    RT R[KBL][NRP];
+    double kq[KBL][4];  // 1/(2q), QC (x,y,z)
+    RT lam[KBL];
+    int sii[KBL];
     RT X[C::IB][S::NCD][S::NHAB];
     HermiteAxis<LA, LB, RT> E[(FACT & 1) ? C::IB : 1][3];
 };
@@ -415,8 +418,8 @@ __device__ __forceinline__ void heavy2_compute(const BuildArgs& A, H2Smem<LA, LB
 }
 
 // Lane-R form of heavy2_compute (see the file header); FACT bit 0 as there.
-template <int LA, int LB, int LC, int LD, typename RT, int FACT>
-__device__ __forceinline__ void heavy2rl_compute(const BuildArgs& A, H2SmemRL<LA, LB, LC, LD, RT, FACT>& sm,
+template <int LA, int LB, int LC, int LD, typename RT, int FACT, bool NARROW>
+__device__ __forceinline__ void heavy2rl_compute(const BuildArgs& A, H2SmemRL<LA, LB, LC, LD, RT, FACT, NARROW>& sm,
                                                  double* sI, int lane, int t0, int u0, int v0, int4 pi, double4 pg,
                                                  int4 qi, double4 qg)
 {
@@ -434,8 +437,9 @@ __device__ __forceinline__ void heavy2rl_compute(const BuildArgs& A, H2SmemRL<LA
             for (int e = lane; e < C::IB * S::NCD * S::NHAB; e += 64) X[e] = RT(0);  // seen after the next barrier
         }
         const int nf = nib * npcd;  // (bra, ket) primitive pairs of this X batch, bra-major
-        for (int f0 = 0; f0 < nf; f0 += C::KBL) {
-            const int nb = nf - f0 < C::KBL ? nf - f0 : C::KBL;
+        constexpr int KBL = H2SmemRL<LA, LB, LC, LD, RT, FACT, NARROW>::KBL;
+        for (int f0 = 0; f0 < nf; f0 += KBL) {
+            const int nb = nf - f0 < KBL ? nf - f0 : KBL;
             __syncthreads();  // readers of the previous R batch are done
             if (lane < nb) {
                 const int f = f0 + lane, ii = f / npcd, kl = f - ii * npcd;

@@ -35,7 +35,8 @@ constexpr double kFp32MaxLog10 = 30.0;  // FP32 quartets only where (2 alpha)^L 
 // 3 = lane per quartet chunked over the ket's first component (64 per unit), 4 = variant 2 with the
 // sum-factorised bra transform (1 per unit), 5 and 6 = variants 2 and 4 with small LDS batches,
 // 7 and 8 = variant 2 with both transforms sum-factorised, big and small LDS batches, 9 and 10 =
-// variants 2 and 4 in lane-R form (whole R tables per lane).
+// variant 2 in lane-R form (whole R tables per lane), R batches of ~16 KB and of ~8 KB with at most
+// 256 registers.
 constexpr int kNumVariants = 11;
 constexpr int kTile[kNumVariants] = {64, 1, 1, 64, 1, 1, 1, 1, 1, 1, 1};
 
@@ -67,7 +68,7 @@ struct ClassPairWork {
         case 7: return info->heavy2k;
         case 8: return info->heavy2ks;
         case 9: return info->heavy2r;
-        default: return info->heavy2rf;
+        default: return info->heavy2rw;
         }
     }
     int unit_list(int v) const { return (v == 0 || v == 3) ? 0 : 1; }

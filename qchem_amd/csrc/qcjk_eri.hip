@@ -17,7 +17,7 @@ constexpr ClassKernelInfo make_info()
             default_light(LA, LB, LC, LD) ? light : &launch_heavy2<LA, LB, LC, LD>, light,
             &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>, lightc, &launch_heavy2f<LA, LB, LC, LD>,
             &launch_heavy2s<LA, LB, LC, LD>, &launch_heavy2fs<LA, LB, LC, LD>, &launch_heavy2k<LA, LB, LC, LD>,
-            &launch_heavy2ks<LA, LB, LC, LD>, &launch_heavy2r<LA, LB, LC, LD>, &launch_heavy2rf<LA, LB, LC, LD>};
+            &launch_heavy2ks<LA, LB, LC, LD>, &launch_heavy2r<LA, LB, LC, LD>, &launch_heavy2rw<LA, LB, LC, LD>};
 }
 #define QCJK_CLASS(LA, LB, LC, LD) make_info<LA, LB, LC, LD>()
 

@@ -67,7 +67,7 @@ struct ClassKernelInfo {
     ClassLauncher heavy2f; // heavy2 with the sum-factorised bra transform
     ClassLauncher heavy2s, heavy2fs;  // heavy2 / heavy2f with small LDS batches
     ClassLauncher heavy2k, heavy2ks;  // heavy2 with both transforms factorised, big / small batches
-    ClassLauncher heavy2r, heavy2rf;  // heavy2 in lane-R form, register / factorised bra transform
+    ClassLauncher heavy2r, heavy2rw;  // heavy2 in lane-R form, R batches of ~16 KB / ~8 KB at <= 256 registers
 };
 
 // Table of the 21 bra-class >= ket-class combinations (qcjk_eri.hip).

@@ -20,9 +20,10 @@
 //   2 k_jk_heavy2  wavefront per quartet, register-resident transforms, batched R (mnd_heavy2.h).
 //   3 k_jk_lightc  lane per quartet, chunked over the ket's first component.
 //   4 k_jk_heavy2  with the sum-factorised bra transform (FACT = true).
-//   5, 6           variants 2 and 4 with small LDS batches (SMALL = true): more wavefronts per SIMD.
+//   5, 6           variants 2 and 4 with small LDS batches (FORM 1): more wavefronts per SIMD.
 //   7, 8           k_jk_heavy2 with both transforms sum-factorised (FACT = 3), big / small batches.
-//   9, 10          k_jk_heavy2 in lane-R form (RL = true), bra transform in registers / factorised.
+//   9, 10          k_jk_heavy2 in lane-R form (FORM 2, 3): ~16 KB R batches / ~8 KB R batches and at
+//                  most 256 registers.
 // With QCJK_MIXED, variants 0, 2 and 3 evaluate small quartets' primitive integrals in FP32
 // (Boys at T >= 40 stays FP64); the contracted block is digested into J/K in FP64.  The FP32 work
 // runs in its own instantiation of the kernel (FP32 = true), launched after the FP64 one over the
@@ -438,14 +439,17 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
 // ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
 // ---------------------------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD, bool FP32, int FACT, bool SMALL, bool RL>
-__global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
+// FORM: 0 level-by-level R batches, 1 the same with small LDS batches, 2 lane-R form, 3 lane-R form
+// with narrow R batches (~8 KB) held to 256 registers (VGPRs + AGPRs): two wavefronts per SIMD.
+template <int LA, int LB, int LC, int LD, bool FP32, int FACT, int FORM>
+__global__ __launch_bounds__(64, (FORM == 3 ? 2 : 1)) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 {
+    constexpr bool SMALL = FORM == 1, RL = FORM >= 2, NARROW = FORM == 3;
     using Dg = HeavyDigest<LA, LB, LC, LD>;
     using S = HeavyShape<LA, LB, LC, LD>;
     using Cf = H2Cfg<LA, LB, LC, LD, SMALL>;
     using RT = std::conditional_t<FP32, float, double>;
-    using SM = std::conditional_t<RL, H2SmemRL<LA, LB, LC, LD, RT, FACT>, H2Smem<LA, LB, LC, LD, RT, FACT, SMALL>>;
+    using SM = std::conditional_t<RL, H2SmemRL<LA, LB, LC, LD, RT, FACT, NARROW>, H2Smem<LA, LB, LC, LD, RT, FACT, SMALL>>;
     __shared__ int rtab[RL ? 1 : S::NR];
     __shared__ double sI[S::NI];
     __shared__ SM sm;
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
         if ((A.flags & kFlagScreen) && !(pg.w * qg.w * (*A.dmax) >= A.screen_tol)) continue;
         const bool small = (A.flags & kFlagMixed) && pg.w * qg.w * (*A.dmax) < A.mixed_tol;  // one quartet per wave
         if (small != FP32) continue;  // the other precision's quartet
-        if constexpr (RL) heavy2rl_compute<LA, LB, LC, LD, RT, FACT>(A, sm, sI, lane, t0, u0, v0, pi, pg, qi, qg);
+        if constexpr (RL) heavy2rl_compute<LA, LB, LC, LD, RT, FACT, NARROW>(A, sm, sI, lane, t0, u0, v0, pi, pg, qi, qg);
         else heavy2_compute<LA, LB, LC, LD, RT, FACT, SMALL>(A, sm, rtab, sI, lane, t0, u0, v0, pi, pg, qi, qg);
         const double s = ((pi.w >> 16) ? 0.5 : 1.0) * ((qi.w >> 16) ? 0.5 : 1.0) * ((C.same && Q == P) ? 0.5 : 1.0);
         for (int t = lane; t < Dg::NTASK; t += 64) {
@@ -530,17 +534,17 @@ hipError_t launch_heavy(const BuildArgs& a, const ClassPairArgs& c, hipStream_t 
 }
 
 // Wavefront-per-quartet launches: FACT selects the factorised transforms (bit 0 bra, bit 1 ket),
-// SMALL the LDS batch sizes.
-template <int LA, int LB, int LC, int LD, int FACT, bool SMALL, bool RL = false>
+// FORM the R batching (as k_jk_heavy2).
+template <int LA, int LB, int LC, int LD, int FACT, int FORM>
 hipError_t launch_heavy2_cfg(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
     if (c.nunits <= 0) return hipSuccess;
     const long long cap = 256LL * 64;
     const long long blocks = c.nunits < cap ? c.nunits : cap;
-    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, FACT, SMALL, RL>), dim3((unsigned)blocks), dim3(64), 0, s,
+    hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, false, FACT, FORM>), dim3((unsigned)blocks), dim3(64), 0, s,
                        a, c);
     if (a.flags & kFlagMixed)
-        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, FACT, SMALL, RL>), dim3((unsigned)blocks), dim3(64), 0,
+        hipLaunchKernelGGL((k_jk_heavy2<LA, LB, LC, LD, true, FACT, FORM>), dim3((unsigned)blocks), dim3(64), 0,
                            s, a, c);
     return hipGetLastError();
 }
@@ -548,49 +552,49 @@ hipError_t launch_heavy2_cfg(const BuildArgs& a, const ClassPairArgs& c, hipStre
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 0, false>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, 0>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2f(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 1, false>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 1, 0>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2s(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 0, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, 1>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2fs(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 1, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 1, 1>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2k(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 3, false>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 3, 0>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2ks(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 3, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 3, 1>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2r(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 0, false, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, 2>(a, c, s);
 }
 
 template <int LA, int LB, int LC, int LD>
-hipError_t launch_heavy2rf(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
+hipError_t launch_heavy2rw(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s)
 {
-    return launch_heavy2_cfg<LA, LB, LC, LD, 1, false, true>(a, c, s);
+    return launch_heavy2_cfg<LA, LB, LC, LD, 0, 3>(a, c, s);
 }
 
 }  // namespace qcjk

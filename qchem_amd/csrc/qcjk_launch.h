@@ -68,6 +68,6 @@ hipError_t launch_heavy2ks(const BuildArgs& a, const ClassPairArgs& c, hipStream
 template <int LA, int LB, int LC, int LD>
 hipError_t launch_heavy2r(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
 template <int LA, int LB, int LC, int LD>
-hipError_t launch_heavy2rf(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
+hipError_t launch_heavy2rw(const BuildArgs& a, const ClassPairArgs& c, hipStream_t s);
 
 }  // namespace qcjk

@@ -4,8 +4,9 @@ workload at hand and cache the choice by (device architecture, class) in a JSON 
 Variants: 0 lane per quartet (register resident), 1 wavefront per quartet with LDS-staged transforms,
 2 wavefront per quartet with register transforms, 3 lane per quartet chunked over the ket's first
 component, 4 variant 2 with the factorised bra transform, 5 and 6 variants 2 and 4 with small LDS
-batches, 7 and 8 variant 2 with both transforms factorised (big and small batches), 9 and 10 variants 2
-and 4 with each lane building whole R tables.  The engine times every variant compiled for a class on the device (qcjk_autotune); this
+batches, 7 and 8 variant 2 with both transforms factorised (big and small batches), 9 and 10 variant 2
+with each lane building whole R tables (R batches of ~16 KB, and of ~8 KB at
+<= 256 registers).  The engine times every variant compiled for a class on the device (qcjk_autotune); this
 module only persists and re-applies the result.
 """
 import json
