@@ -50,6 +50,7 @@ struct H2Cfg {
     // or (narrow) about 8 KB
     static constexpr int KBL = clampi(16384 / (8 * S::NR), 16, 64);
     static constexpr int KBLN = clampi(8192 / (8 * (S::NR | 1)), 8, 64);
+    static constexpr int IBN = clampi(1024 / XI, 1, IB0);
     static constexpr int GL = 64 / S::NHAB < 1 ? 1 : 64 / S::NHAB;
 };
 
@@ -74,13 +75,14 @@ struct H2SmemRL {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD, false>;
     static constexpr int KBL = NARROW ? C::KBLN : C::KBL;  // slots per R batch
+    static constexpr int IB = NARROW ? C::IBN : C::IB;     // bra primitive pairs per X batch
     static constexpr int NRP = S::NR | 1;                // odd row length
     RT R[KBL][NRP];
     double kq[KBL][4];  // 1/(2q), QC (x,y,z)
     RT lam[KBL];
     int sii[KBL];
-    RT X[C::IB][S::NCD][S::NHAB];
-    HermiteAxis<LA, LB, RT> E[(FACT & 1) ? C::IB : 1][3];
+    RT X[IB][S::NCD][S::NHAB];
+    HermiteAxis<LA, LB, RT> E[(FACT & 1) ? IB : 1][3];
 };
 
 // Reference-order index of the Cartesian component (x, y, z) of a shell of angular momentum l.
@@ -425,19 +427,20 @@ __device__ __forceinline__ void heavy2rl_compute(const BuildArgs& A, H2SmemRL<LA
 {
     using S = HeavyShape<LA, LB, LC, LD>;
     using C = H2Cfg<LA, LB, LC, LD, false>;
+    using SMT = H2SmemRL<LA, LB, LC, LD, RT, FACT, NARROW>;
     constexpr int NHCD = nherm(S::LCD);
     for (int e = lane; e < S::NI; e += 64) sI[e] = 0.0;
     const int npab = pi.w & 0xffff, npcd = qi.w & 0xffff;
     const bool xl = lane < C::GL * S::NHAB;
     const int grp = lane / S::NHAB, hl = lane - grp * S::NHAB;  // ket-transform group, Hermite index
-    for (int ij0 = 0; ij0 < npab; ij0 += C::IB) {
-        const int nib = npab - ij0 < C::IB ? npab - ij0 : C::IB;
+    for (int ij0 = 0; ij0 < npab; ij0 += SMT::IB) {
+        const int nib = npab - ij0 < SMT::IB ? npab - ij0 : SMT::IB;
         {
             RT* X = &sm.X[0][0][0];
-            for (int e = lane; e < C::IB * S::NCD * S::NHAB; e += 64) X[e] = RT(0);  // seen after the next barrier
+            for (int e = lane; e < SMT::IB * S::NCD * S::NHAB; e += 64) X[e] = RT(0);  // seen after the next barrier
         }
         const int nf = nib * npcd;  // (bra, ket) primitive pairs of this X batch, bra-major
-        constexpr int KBL = H2SmemRL<LA, LB, LC, LD, RT, FACT, NARROW>::KBL;
+        constexpr int KBL = SMT::KBL;
         for (int f0 = 0; f0 < nf; f0 += KBL) {
             const int nb = nf - f0 < KBL ? nf - f0 : KBL;
             __syncthreads();  // readers of the previous R batch are done

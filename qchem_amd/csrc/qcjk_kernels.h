@@ -95,17 +95,33 @@ __device__ __forceinline__ bool wave_fp32(const BuildArgs& A, bool active, doubl
     return __ballot(big) == 0;
 }
 
-__device__ __forceinline__ void count_lane_quartets(const BuildArgs& A, bool active, bool fp32, int lane, int4 pi,
-                                                    int4 qi)
-{
-    const unsigned long long m = __ballot(active);
-    if (lane == 0 && m) {
-        const int cnt = __popcll(m);
-        atomicAdd(&A.counters[0], (unsigned long long)cnt);
-        if (fp32) atomicAdd(&A.counters[1], (unsigned long long)cnt);
-        atomicAdd(&A.counters[2], (unsigned long long)cnt * (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
+// Per-wavefront tallies of the quartets computed (all, FP32, primitive quartets), added to the
+// class's counters once when the wavefront ends: atomics per quartet on three global words would
+// queue in one L2 channel.  Primitive counts use lane 0's contraction lengths.
+struct WaveTally {
+    unsigned long long q = 0, q32 = 0, prim = 0;
+    __device__ __forceinline__ void lanes(bool active, bool fp32, int4 pi, int4 qi)  // every lane calls
+    {
+        const unsigned long long cnt = (unsigned long long)__popcll(__ballot(active));
+        q += cnt;
+        if (fp32) q32 += cnt;
+        prim += cnt * (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff));
     }
-}
+    __device__ __forceinline__ void one(bool fp32, int4 pi, int4 qi)
+    {
+        q += 1;
+        if (fp32) q32 += 1;
+        prim += (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff));
+    }
+    __device__ __forceinline__ void flush(const BuildArgs& A, int lane) const
+    {
+        if (A.counters && lane == 0 && q) {
+            atomicAdd(&A.counters[0], q);
+            if (q32) atomicAdd(&A.counters[1], q32);
+            atomicAdd(&A.counters[2], prim);
+        }
+    }
+};
 
 // ---------------------------------------------------------------------------------------------
 // Lane-per-quartet kernel
@@ -118,6 +134,7 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
     using RT = std::conditional_t<FP32, float, double>;
     __shared__ HermiteAxis<LA, LB, RT> sEab[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
+    WaveTally tally;
     for (long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); unit < C.nunits;
          unit += (long long)gridDim.x * 4) {  // uniform per wavefront
         const int P = find_unit_owner(C.unit_start, C.nbra, unit);
@@ -252,8 +269,9 @@ __global__ __launch_bounds__(256) void k_jk_light(BuildArgs A, ClassPairArgs C)
                 });
             }
         }
-        if (A.counters) count_lane_quartets(A, active, FP32, lane, pi, qi);
+        tally.lanes(active, FP32, pi, qi);
     }
+    tally.flush(A, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -269,6 +287,7 @@ __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
     using RT = std::conditional_t<FP32, float, double>;
     __shared__ HermiteAxis<LA, LB, RT> sEab[BRA_MEM ? 4 : 1][3];
     const int lane = threadIdx.x & 63;
+    WaveTally tally;
     for (long long unit = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); unit < C.nunits;
          unit += (long long)gridDim.x * 4) {  // uniform per wavefront
         const int P = find_unit_owner(C.unit_start, C.nbra, unit);
@@ -391,8 +410,9 @@ __global__ __launch_bounds__(256) void k_jk_lightc(BuildArgs A, ClassPairArgs C)
                 unsafeAtomicAdd(&Kt[(ob + b) * n + od + d], s * kbd[decltype(x)::value]);
             });
         }
-        if (A.counters) count_lane_quartets(A, active, FP32, lane, pi, qi);
+        tally.lanes(active, FP32, pi, qi);
     }
+    tally.flush(A, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -409,6 +429,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
     HeavyLane<LA, LB, LC, LD> ln;
     ln.init(lane);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    WaveTally tally;
     UnitRange ur(C);
     for (long long u = ur.u; u < ur.end; ++u) {
         const int P = ur.owner(C, u);
@@ -429,18 +450,17 @@ __global__ __launch_bounds__(64) void k_jk_heavy(BuildArgs A, ClassPairArgs C)
                 unsafeAtomicAdd(&tgt[row * mm.y + col], val);
             }
         }
-        if (A.counters && lane == 0) {
-            atomicAdd(&A.counters[0], 1ull);
-            atomicAdd(&A.counters[2], (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
-        }
+        tally.one(false, pi, qi);
     }
+    tally.flush(A, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Wavefront-per-quartet kernel with register-resident transforms (mnd_heavy2.h)
 // ---------------------------------------------------------------------------------------------
 // FORM: 0 level-by-level R batches, 1 the same with small LDS batches, 2 lane-R form, 3 lane-R form
-// with narrow R batches (~8 KB) held to 256 registers (VGPRs + AGPRs): two wavefronts per SIMD.
+// with narrow R and X batches (~8 KB each) held to 256 registers (VGPRs + AGPRs): two wavefronts per
+// SIMD.
 template <int LA, int LB, int LC, int LD, bool FP32, int FACT, int FORM>
 __global__ __launch_bounds__(64, (FORM == 3 ? 2 : 1)) void k_jk_heavy2(BuildArgs A, ClassPairArgs C)
 {
@@ -458,6 +478,7 @@ __global__ __launch_bounds__(64, (FORM == 3 ? 2 : 1)) void k_jk_heavy2(BuildArgs
     int t0 = 0, u0 = 0, v0 = 0;
     if (lane < (RL ? Cf::GL : Cf::G) * S::NHAB) hdec(lane % S::NHAB, t0, u0, v0);
     const bool doJ = (A.flags & kFlagJ) != 0, doK = (A.flags & kFlagK) != 0;
+    WaveTally tally;
     UnitRange ur(C);
     for (long long u = ur.u; u < ur.end; ++u) {
         const int P = ur.owner(C, u);
@@ -481,13 +502,10 @@ __global__ __launch_bounds__(64, (FORM == 3 ? 2 : 1)) void k_jk_heavy2(BuildArgs
                 unsafeAtomicAdd(&tgt[row * mm.y + col], val);
             }
         }
-        if (A.counters && lane == 0) {
-            atomicAdd(&A.counters[0], 1ull);
-            if (FP32) atomicAdd(&A.counters[1], 1ull);
-            atomicAdd(&A.counters[2], (unsigned long long)((pi.w & 0xffff) * (qi.w & 0xffff)));
-        }
+        tally.one(FP32, pi, qi);
         __syncthreads();  // digest reads of sI finish before the next quartet clears it
     }
+    tally.flush(A, lane);
 }
 
 // ---------------------------------------------------------------------------------------------
