@@ -491,7 +491,7 @@ static int create_impl(const qcjk_basis* mols, int nmol, const qcjk_opts* o, qcj
             w.ket = kc;
             w.info = class_kernel(bc, kc);
             if (!w.info) return QCJK_ERR_UNSUPPORTED;
-            w.variant = force_heavy ? 1 : ((w.info->tile == 64 && w.info->light) ? 0 : 2);
+            w.variant = force_heavy ? 1 : ((w.info->tile == 64 && w.info->light) ? 0 : 10);
             {
                 // FP32 range guard: R^{(0)}_{tuv} carries up to (2 alpha)^L with alpha <= min(p, q).
                 const int L = w.info->la + w.info->lb + w.info->lc + w.info->ld;

@@ -16,7 +16,7 @@ constexpr bool has_light(int la, int lb, int lc, int ld)
     return ncart_l(la) * ncart_l(lb) * ncart_l(lc) * ncart_l(ld) <= 81 && la + lb + lc + ld <= 4;
 }
 // Default choice before tuning (measured on the random-cluster workload): lane per quartet for
-// L <= 4 except (dd|ss), the register-transform wavefront kernel otherwise.
+// L <= 4 except (dd|ss), the lane-R wavefront kernel with narrow batches (variant 10) otherwise.
 constexpr bool default_light(int la, int lb, int lc, int ld)
 {
     return has_light(la, lb, lc, ld) && !(la == 2 && lb == 2);
