@@ -13,8 +13,7 @@ constexpr ClassKernelInfo make_info()
     ClassLauncher light = nullptr, lightc = nullptr;
     if constexpr (has_light(LA, LB, LC, LD)) light = &launch_light<LA, LB, LC, LD>;
     if constexpr (has_lightc(LA, LB, LC, LD)) lightc = &launch_lightc<LA, LB, LC, LD>;
-    return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1,
-            default_light(LA, LB, LC, LD) ? light : &launch_heavy2rw<LA, LB, LC, LD>, light,
+    return {LA, LB, LC, LD, default_light(LA, LB, LC, LD) ? 64 : 1, light,
             &launch_heavy<LA, LB, LC, LD>, &launch_heavy2<LA, LB, LC, LD>, lightc, &launch_heavy2f<LA, LB, LC, LD>,
             &launch_heavy2s<LA, LB, LC, LD>, &launch_heavy2fs<LA, LB, LC, LD>, &launch_heavy2k<LA, LB, LC, LD>,
             &launch_heavy2ks<LA, LB, LC, LD>, &launch_heavy2r<LA, LB, LC, LD>, &launch_heavy2rw<LA, LB, LC, LD>};

@@ -54,12 +54,12 @@ struct ClassPairArgs {
 // Launch one class-pair kernel.  Returns hipError_t.
 typedef hipError_t (*ClassLauncher)(const BuildArgs&, const ClassPairArgs&, hipStream_t);
 
-// Kernel variants of one class: lane per quartet (tile 64, absent for large classes) and two
-// wavefront-per-quartet kernels (tile 1, always present).  tile/launch are the untuned default.
+// Kernel variants of one class: lane per quartet (tile 64, absent for large classes) and the
+// wavefront-per-quartet kernels (tile 1, always present).  tile 64 marks the classes whose untuned
+// default is the lane-per-quartet kernel.
 struct ClassKernelInfo {
     int la, lb, lc, ld;
     int tile;
-    ClassLauncher launch;
     ClassLauncher light;   // may be null
     ClassLauncher heavy;   // LDS-staged transforms (mnd_heavy.h)
     ClassLauncher heavy2;  // register-resident transforms (mnd_heavy2.h)
