@@ -700,17 +700,21 @@ static int enqueue_build(qcjk_handle* h, const double* dD, double* dJ, double* d
     if (flags & (QCJK_J | QCJK_K)) {
         // Class launches round-robin over ns streams forked from and joined back into s, so one
         // class's tail overlaps the next class's start (every kernel adds into Jt/Kt atomically).
+        // The classes are taken alternately from the light and the heavy end of the class list,
+        // so each stream mixes latency-bound lane kernels with issue-bound wavefront kernels.
         // Per-class profiling events go on the class's own stream.
         const int ns = h->nstreams;
+        const size_t nw = h->work.size();
         if (ns > 1) {
             HIPCHK(hipEventRecord(h->fork, s));
             for (int k = 0; k < ns - 1; ++k) HIPCHK(hipStreamWaitEvent(h->side[k], h->fork, 0));
         }
         BuildArgs a = make_args(h, flags);
-        for (size_t i = 0; i < h->work.size(); ++i) {
+        for (size_t j = 0; j < nw; ++j) {
+            const size_t i = (ns > 1) ? ((j & 1) ? nw - 1 - j / 2 : j / 2) : j;
             const ClassPairWork& w = h->work[i];
             const ClassPairArgs c = class_args(h, w, w.variant);
-            const int k = (int)(i % (size_t)ns);
+            const int k = (int)(j % (size_t)ns);
             hipStream_t si = k == 0 ? s : h->side[k - 1];
             a.counters = h->d_counters + 3 * i;
             a.flags = w.fp32_ok ? flags : (flags & ~(unsigned)QCJK_MIXED);
