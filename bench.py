@@ -198,6 +198,8 @@ def main():
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF, mixed-precision and batched sections")
     ap.add_argument("--no-tune", action="store_true", help="use the untuned default kernel variant per class")
     ap.add_argument("--retune", action="store_true", help="re-run the per-class kernel-variant autotuner")
+    ap.add_argument("--tune-out", default=os.path.join(ROOT, "tuning_retuned.json"),
+                    help="where --retune writes the new per-class table")
     ap.add_argument("--batch-scf", action="store_true", help="also run the 512-molecule batched RHF (minutes)")
     ap.add_argument("--streams", type=int, default=0,
                     help="concurrent class streams per build (1-4; 0 = the engine's default)")
@@ -244,8 +246,7 @@ def main():
     from qchem_amd import tune
     workload_key = "random-cluster-16"
     if args.retune:  # untimed setup: time every kernel variant per class on this workload
-        rep = tune.tune(eng, D, reps=2, cache=os.path.join(ROOT, "gpurun_out", "tuning_gfx950.json"),
-                        workload=workload_key)
+        rep = tune.tune(eng, D, reps=2, cache=args.tune_out, workload=workload_key)
         tuning = {"source": "autotuned now", "classes": {"%d%d%d%d" % (r["la"], r["lb"], r["lc"], r["ld"]): [
             r["wavefront_per_quartet"], round(r["device_ms"], 3), round(r["alt_ms"], 3)] for r in rep}}
     elif not args.no_tune:  # the committed per-class choices for this workload on gfx950
