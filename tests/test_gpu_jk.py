@@ -186,6 +186,36 @@ def test_class_streams_agree(require_gpu):
     assert qa.JKEngine(B).streams() == 3  # the default
 
 
+def long_contraction_molecule(seed):
+    """Two centres, each with s (K=12), p (K=9), d (K=7) and an uncontracted s shell: primitive
+    batches that span several R batches, uneven contraction lengths within one class."""
+    rng = np.random.default_rng(seed)
+    shells = []
+    for _ in range(2):
+        c = list(rng.uniform(-1.2, 1.2, 3))
+        for L, K in ((0, 12), (1, 9), (2, 7), (0, 1)):
+            shells.append(dict(L=L, exps=list(np.exp(rng.uniform(np.log(0.1), np.log(50.0), K))),
+                               coefs=list(rng.uniform(0.2, 1.0, K)), center=c, contracted=True))
+    return shells
+
+
+def test_long_contractions(require_gpu):
+    """The wavefront kernels (level-by-level R batches and both lane-R forms) on long and uneven
+    contractions, every class, against the oracle."""
+    shells = long_contraction_molecule(83)
+    B = qa.Basis(shells)
+    D = rand_density(B.nbf, 83)
+    Jr, Kr, _ = oracle.Basis(shells).jk_direct(D)
+    classes = [(a, b, c, d) for a in range(3) for b in range(a + 1) for c in range(3) for d in range(c + 1)]
+    for variant in (2, 9, 10):
+        eng = qa.JKEngine(B)
+        for cls in classes:
+            eng.set_variant(*cls, variant)
+        J, K, _ = eng.build(D)
+        print("variant %d: J err %.2e K err %.2e" % (variant, rel_err(J, Jr), rel_err(K, Kr)))
+        assert rel_err(J, Jr) < TOL and rel_err(K, Kr) < TOL, variant
+
+
 def test_autotune_keeps_results(require_gpu):
     import torch
     m = small_cluster(6, 61)
